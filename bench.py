@@ -1,0 +1,188 @@
+"""Benchmark of the Wilson-flow path on one MI355X.
+
+Workload (the single-lattice configuration of the headline metric): one
+24^4 lattice at beta = 6.0, Luescher RK3 with eps = 0.01, E, t^2E and Q
+measured every 5 steps.  A "step" is one RK3 step of the whole lattice
+(4*24^4 link-updates) plus, every 5th step, one clover measurement.
+
+Reported `value`: link-updates/s of the FP32-arithmetic path with FP64
+reductions and FP64 measurements, whole job (flow + measurements), links
+resident in HBM.  Also reported: the same for the FP64 parity path, the
+flow-only rates, the roofline of the RK3 stage kernels (HIP events on the
+library's stream) and a CPU baseline (the CPU oracle on a bounded sample).
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+SHAPE = (24, 24, 24, 24)
+EPS = 0.01
+MEASURE_EVERY = 5
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_LU = {0: 1440.0, 1: 720.0}  # (6 s_U + 4 s_X) * 18 bytes per link-update
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream (the library's own)."""
+
+    def __init__(self):
+        self.h = ctypes.CDLL("libamdhip64.so")
+        self.h.hipEventCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+        self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
+        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
+
+    def event(self):
+        e = ctypes.c_void_p()
+        assert self.h.hipEventCreate(ctypes.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.h.hipEventRecord(e, ctypes.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b):
+        assert self.h.hipEventSynchronize(b) == 0
+        ms = ctypes.c_float()
+        assert self.h.hipEventElapsedTime(ctypes.byref(ms), a, b) == 0
+        return ms.value
+
+
+def make_input(seed=1234):
+    """Seeded Haar-random links (no thermalisation): the flow's cost does
+    not depend on link values except through the exponential's branch mix."""
+    from oracle import oracle as O
+    return O.generate(SHAPE, 6.0, seed=seed, start="hot", sweeps=0)
+
+
+def run_path(lgf, ctx, U, prec, steps, warmup, ev):
+    f = lgf.Field(ctx, SHAPE, prec)
+    f.upload(U)
+    b = lgf.Batch(ctx, [f])
+    flags = lgf.MEASURE_FP64
+    # warmup (also captures the graphs)
+    for s in range(warmup):
+        b.flow(EPS, 1, use_graph=True)
+    b.flow(EPS, 2, use_graph=True)
+    b.measure(flags)
+    ctx.synchronize()
+    stream = ctx.stream()
+    # flow-only timing of the stage kernels with HIP events
+    e0, e1 = ev.event(), ev.event()
+    nflow = max(2, steps - steps % 2)
+    ev.record(e0, stream)
+    b.flow(EPS, nflow, use_graph=True)
+    ev.record(e1, stream)
+    flow_ms = ev.elapsed_ms(e0, e1) / nflow
+    # one measurement
+    m0, m1 = ev.event(), ev.event()
+    ev.record(m0, stream)
+    b.measure(flags)
+    ev.record(m1, stream)
+    meas_ms = ev.elapsed_ms(m0, m1)
+    # whole job: steps with a measurement every MEASURE_EVERY steps
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    done = 0
+    while done < steps:
+        n = min(MEASURE_EVERY, steps - done)
+        b.flow(EPS, n, use_graph=True)
+        done += n
+        if done % MEASURE_EVERY == 0:
+            b.measure(flags)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    E, Q, Qa = b.measure(flags)
+    diag = f.diagnostics()
+    b.close()
+    f.close()
+    return {"wall_s": wall, "flow_ms_per_step": flow_ms, "measure_ms": meas_ms, "E": float(E[0]),
+            "Q": float(Q[0]), "unitarity_max": diag[1]}
+
+
+def cpu_baseline(U, budget_s=20.0):
+    """The CPU oracle (a scalar-per-thread C port of the reference path) on
+    whole RK3 steps of the same 24^4 lattice, within a time budget."""
+    from oracle import oracle as O
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    V = U.copy()
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        O.rk3_step(SHAPE, V, EPS)
+        n += 1
+        if time.perf_counter() - t0 > budget_s / 2 or n >= 4:
+            break
+    dt = time.perf_counter() - t0
+    lu = 4.0 * np.prod(SHAPE) * n
+    return {"value": lu / dt, "unit": "link-updates/s", "cores": threads, "kind": "port",
+            "sample": "%d RK3 step(s) of the 24^4 lattice with the CPU oracle (oracle/lgf_oracle.c, "
+                      "OpenMP over sites), no measurements, %.1f s" % (n, dt)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the flow path runs one lattice (or one batch) per GPU "
+              "and is not sharded across GPUs; run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    from msc_project_amd import lgf
+    ev = HipEvents()
+    U = make_input()
+    ctx = lgf.Context(0)
+    res = {}
+    for prec in (lgf.FP64, lgf.FP32):
+        res[prec] = run_path(lgf, ctx, U, prec, args.steps, args.warmup, ev)
+        print("# %s: %s" % ("fp64" if prec == 0 else "fp32", json.dumps(res[prec])), file=sys.stderr, flush=True)
+    lu_step = 4.0 * np.prod(SHAPE)
+    r32, r64 = res[lgf.FP32], res[lgf.FP64]
+    value = lu_step * args.steps / r32["wall_s"]
+    achieved = BYTES_PER_LU[1] * lu_step / (r32["flow_ms_per_step"] * 1e-3) / 1e9
+    line = {
+        "metric": "RK3 flow link-updates/s (FP32 arithmetic, FP64 reductions) on one 24^4 lattice",
+        "value": value,
+        "unit": "link-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": r32["wall_s"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: seeded Haar-random 24^4 SU(3) links",
+        "config": {"workload": "single 24^4 lattice, beta 6.0, RK3 eps 0.01, E/t^2E/Q every 5 steps",
+                   "lattice": list(SHAPE), "eps": EPS, "measure_every": MEASURE_EVERY},
+        "fp32_flow_only_lups": lu_step / (r32["flow_ms_per_step"] * 1e-3),
+        "fp64_lups": lu_step * args.steps / r64["wall_s"],
+        "fp64_flow_only_lups": lu_step / (r64["flow_ms_per_step"] * 1e-3),
+        "fp32_over_fp64_flow": r64["flow_ms_per_step"] / r32["flow_ms_per_step"],
+        "measure_ms": {"fp32_links": r32["measure_ms"], "fp64_links": r64["measure_ms"]},
+        "fp32_vs_fp64_final": {"dE_rel": abs(r32["E"] - r64["E"]) / abs(r64["E"]), "dQ": abs(r32["Q"] - r64["Q"])},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_rk3_stage<float> x3 per step (720 B per link-update algorithmic)"},
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(U)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

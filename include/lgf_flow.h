@@ -1,0 +1,160 @@
+/*
+ * lgf_flow.h -- C-ABI of the MI355X (gfx950) Wilson-flow and clover-observable
+ * path.  Plain pointers and sizes only; every call returns an lgf_status and,
+ * on failure, leaves a message retrievable with lgf_ctx_last_error().
+ *
+ * The reference (LGF) has no FFI layer of its own; these entry points replace
+ * the CPU work underneath its C++ host API:
+ *
+ *   lgf_field_upload / lgf_field_download
+ *       the link storage of SU3_field after loadSU3FromFile / before
+ *       saveSU3ToFile (LGF/Field/SU3_field.cpp:260-352, 89-172); the buffer is
+ *       in the reference's saved-configuration order, links[((g*4+mu)*9+k)*2+ri].
+ *   lgf_flow_steps
+ *       GradientFlow::flow() (LGF/Gradient_Flow/GradientFlow.cpp:22-60) with
+ *       Wilson::calculate_FlowGradient (LGF/Action/Gauge/Wilson.cpp:55-71)
+ *       and HermTrLessExp (LGF/Math/SU3_mat.cpp:216-289).
+ *   lgf_measure
+ *       EnergyDensity::calculate (LGF/Observables/EnergyDensity.cpp:9-34),
+ *       TopologicalCharge::calculate (LGF/Observables/TopologicalCharge.cpp:8-29)
+ *       and the per-site sum of QdensityDistribution::calculate
+ *       (LGF/Observables/QdensityDistribution.cpp:15-28).
+ *   lgf_diagnostics
+ *       SU3_field::Avg_Plaquette (LGF/Field/SU3_field.cpp:485-499) and the
+ *       unitarity check of isSpecialUnitary (LGF/Math/SU3_mat.cpp:420-454).
+ *   lgf_batch_*
+ *       the reference's loop over configurations in
+ *       FlowSavedGaugeConfigurations (LGF/LGF.cpp:169-195), done as one set of
+ *       launches over all configurations of a batch.
+ *   lgf_server_*
+ *       no counterpart in the reference (a batch CLI tool); a continuous-
+ *       batching request loop for flow jobs.
+ *
+ * Threading: calls on one context are not reentrant; use one context per
+ * host thread.  The server runs its own scheduler thread and is safe to
+ * submit to from any thread.
+ */
+#ifndef LGF_FLOW_H
+#define LGF_FLOW_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lgf_ctx lgf_ctx;
+typedef struct lgf_field lgf_field;
+typedef struct lgf_batch lgf_batch;
+typedef struct lgf_server lgf_server;
+
+enum lgf_status {
+    LGF_OK = 0,
+    LGF_E_ARG = 1,       /* null pointer, bad enum, bad count */
+    LGF_E_SHAPE = 2,     /* odd or non-positive extent, volume too large */
+    LGF_E_IO = 3,        /* file missing, short or unwritable */
+    LGF_E_NOMEM = 4,     /* device or host allocation failed */
+    LGF_E_HIP = 5,       /* HIP runtime error */
+    LGF_E_NAN = 6,       /* non-finite observable */
+    LGF_E_STATE = 7,     /* object used in an invalid state */
+    LGF_E_ADMISSION = 8, /* request larger than the server's memory budget */
+    LGF_E_UNSUPPORTED = 9
+};
+
+enum lgf_precision {
+    LGF_FP64 = 0, /* FP64 links and arithmetic: the parity path */
+    LGF_FP32 = 1  /* FP32 links and arithmetic, FP64 reductions, FP64 reunitarisation per step */
+};
+
+enum lgf_measure_flags {
+    LGF_MEASURE_DEFAULT = 0, /* clover arithmetic in the field's precision */
+    LGF_MEASURE_FP64 = 1     /* clover arithmetic in FP64 also for FP32 fields */
+};
+
+/* ---- context ---------------------------------------------------------- */
+int lgf_ctx_create(int device, lgf_ctx** out);
+int lgf_ctx_destroy(lgf_ctx* ctx);
+const char* lgf_ctx_last_error(const lgf_ctx* ctx);
+/* the hipStream_t every call of this context is ordered on */
+int lgf_ctx_stream(lgf_ctx* ctx, void** hip_stream);
+int lgf_ctx_synchronize(lgf_ctx* ctx);
+/* block shape choice for the stage kernel: 0 = tuned default; see lgf_autotune */
+int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
+int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
+int lgf_num_stage_variants(int precision, int* n);
+
+/* ---- one lattice ------------------------------------------------------ */
+/* device bytes held by a field of this shape and precision */
+int lgf_field_footprint(const int shape[4], int precision, size_t* bytes);
+int lgf_field_create(lgf_ctx* ctx, const int shape[4], int precision, lgf_field** out);
+int lgf_field_destroy(lgf_field* f);
+int lgf_field_shape(const lgf_field* f, int shape[4], int* precision);
+/* links: volume*4*18 doubles in the reference's saved-configuration order */
+int lgf_field_upload(lgf_field* f, const double* links);
+int lgf_field_download(lgf_field* f, double* links);
+int lgf_field_set_cold(lgf_field* f);
+/* copy links of src into dst (same shape; precision converted) */
+int lgf_field_copy(lgf_field* dst, const lgf_field* src);
+/* read/write a configuration file of the reference's format at `path`
+ * (headerless, 144 B per link, 64-bit offsets) */
+int lgf_field_load_file(lgf_field* f, const char* path);
+int lgf_field_save_file(lgf_field* f, const char* path);
+
+/* nsteps RK3 steps of size eps */
+int lgf_flow_steps(lgf_field* f, double eps, int nsteps);
+/* E = (1/V) sum_x sum_{mu<nu} ReTr(F_munu F_munu) with the 1x1 clover F;
+ * Q = 1/(4 pi^2) sum_x qraw(x) with the improved F;
+ * Qabs = 1/(4 pi^2) sum_x |qraw(x)| (a scale for judging Q differences);
+ * qraw (optional, volume doubles at the global index) =
+ *   ReTr(F32 F01) + ReTr(F31 F20) + ReTr(F30 F12) per site. */
+int lgf_measure(lgf_field* f, int flags, double* E, double* Q, double* Qabs, double* qraw);
+int lgf_diagnostics(lgf_field* f, double* avg_plaquette, double* unitarity_max, double* unitarity_mean);
+
+/* ---- many lattices (any shapes) in one set of launches ----------------- */
+int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfields, lgf_batch** out);
+int lgf_batch_destroy(lgf_batch* b);
+/* eps: one value per field.  use_graph != 0 replays the RK3 steps from a
+ * captured hipGraph (two steps per graph). */
+int lgf_batch_flow_steps(lgf_batch* b, const double* eps, int nsteps, int use_graph);
+int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, double* Qabs);
+
+/* ---- continuous-batching flow server ----------------------------------- */
+typedef struct {
+    int shape[4];
+    int precision;          /* lgf_precision */
+    double beta;            /* carried into the result, not used by the flow */
+    double eps;
+    int nsteps;
+    int measure_every;      /* measure at t=0 and every k steps; 0 = only at t=0 and the end */
+    const double* links;    /* initial links in file order (copied at submit) or NULL for cold start */
+    int measure_flags;      /* lgf_measure_flags */
+} lgf_request;
+
+typedef struct {
+    uint64_t id;
+    int status;             /* lgf_status of the job */
+    int nmeas;
+    double submit_s, start_s, finish_s; /* seconds on the server's steady clock */
+    double t[1024];         /* flow times of the measurements (first nmeas entries) */
+    double E[1024];
+    double Q[1024];
+} lgf_result;
+
+/* budget_bytes: device memory the server may hold for jobs (0 = 80% of free);
+ * max_active: jobs flowing at once (0 = unlimited within the budget) */
+int lgf_server_create(int device, size_t budget_bytes, int max_active, lgf_server** out);
+int lgf_server_destroy(lgf_server* s);
+int lgf_server_submit(lgf_server* s, const lgf_request* req, uint64_t* id);
+/* wait (timeout_s < 0: forever) for the job; LGF_E_STATE on timeout */
+int lgf_server_wait(lgf_server* s, uint64_t id, double timeout_s, lgf_result* out);
+int lgf_server_now(lgf_server* s, double* now_s);
+/* counters: jobs done, steps executed, link-updates executed, busy seconds */
+int lgf_server_stats(lgf_server* s, uint64_t* jobs_done, uint64_t* batch_steps, double* link_updates,
+                     double* busy_s);
+const char* lgf_server_last_error(const lgf_server* s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LGF_FLOW_H */

@@ -1,0 +1,549 @@
+// C-ABI of the gfx950 Wilson-flow path (declarations and the reference
+// interface each call replaces: include/lgf_flow.h).
+//
+// Ownership: a context owns a HIP stream and a device staging buffer; a field
+// owns its two ping-pong link buffers and the RK3 accumulator; a batch owns a
+// device table of field descriptors, the tile map and the reduction buffers
+// but not its fields.  Nothing here falls back to the CPU: every compute
+// call launches the kernels of kernels.hip or returns an error.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lgf_flow.h"
+#include "capi_internal.hpp"
+#include "kernels.hpp"
+
+using namespace lgf;
+
+namespace {
+
+constexpr double kPi = 3.141592653589793238462643383279502884;
+constexpr size_t kStageBytes = size_t(64) << 20;  // host<->device staging chunk
+
+int64_t round_stride(int64_t vol) { return (vol + 63) / 64 * 64; }
+
+int check_shape(const int* shape, int64_t* vol) {
+    if (!shape) return LGF_E_ARG;
+    int64_t v = 1;
+    for (int i = 0; i < 4; ++i) {
+        if (shape[i] <= 0 || (shape[i] & 1)) return LGF_E_SHAPE;  // even extents (LGF/Lattice/Lattice.cpp:42-49)
+        v *= shape[i];
+    }
+    if (v * 4 * 18 >= (int64_t(1) << 62) || v >= (int64_t(1) << 31) - 64) return LGF_E_SHAPE;
+    *vol = v;
+    return LGF_OK;
+}
+
+size_t elem_size(int prec) { return prec == LGF_FP64 ? sizeof(double) : sizeof(float); }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// error plumbing
+// ---------------------------------------------------------------------------
+int lgf_fail(lgf_ctx* ctx, int code, const std::string& msg) {
+    if (ctx) ctx->err = msg;
+    return code;
+}
+
+int lgf_hip_fail(lgf_ctx* ctx, hipError_t e, const char* what) {
+    return lgf_fail(ctx, LGF_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIPTRY(ctx, expr)                                    \
+    do {                                                     \
+        hipError_t _e = (expr);                              \
+        if (_e != hipSuccess) return lgf_hip_fail(ctx, _e, #expr); \
+    } while (0)
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+extern "C" int lgf_ctx_create(int device, lgf_ctx** out) {
+    if (!out) return LGF_E_ARG;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return LGF_E_HIP;
+    if (device < 0 || device >= n) return LGF_E_ARG;
+    if (hipSetDevice(device) != hipSuccess) return LGF_E_HIP;
+    lgf_ctx* c = new (std::nothrow) lgf_ctx();
+    if (!c) return LGF_E_NOMEM;
+    c->device = device;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return LGF_E_HIP;
+    }
+    *out = c;
+    return LGF_OK;
+}
+
+extern "C" int lgf_ctx_destroy(lgf_ctx* ctx) {
+    if (!ctx) return LGF_E_ARG;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->staging) (void)hipFree(ctx->staging);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return LGF_OK;
+}
+
+extern "C" const char* lgf_ctx_last_error(const lgf_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+extern "C" int lgf_ctx_stream(lgf_ctx* ctx, void** s) {
+    if (!ctx || !s) return LGF_E_ARG;
+    *s = (void*)ctx->stream;
+    return LGF_OK;
+}
+
+extern "C" int lgf_ctx_synchronize(lgf_ctx* ctx) {
+    if (!ctx) return LGF_E_ARG;
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LGF_OK;
+}
+
+extern "C" int lgf_num_stage_variants(int precision, int* n) {
+    if (!n || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
+    *n = num_stage_variants(precision);
+    return LGF_OK;
+}
+
+extern "C" int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant) {
+    if (!ctx || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
+    if (variant < 0 || variant >= num_stage_variants(precision))
+        return lgf_fail(ctx, LGF_E_ARG, "stage variant out of range");
+    ctx->stage_variant[precision] = variant;
+    return LGF_OK;
+}
+
+extern "C" int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant) {
+    if (!ctx || !variant || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
+    *variant = ctx->stage_variant[precision];
+    return LGF_OK;
+}
+
+int ctx_staging(lgf_ctx* ctx, void** dev, void** host) {
+    if (!ctx->staging) HIPTRY(ctx, hipMalloc(&ctx->staging, kStageBytes));
+    if (!ctx->pinned) HIPTRY(ctx, hipHostMalloc(&ctx->pinned, kStageBytes, hipHostMallocDefault));
+    *dev = ctx->staging;
+    *host = ctx->pinned;
+    return LGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// fields
+// ---------------------------------------------------------------------------
+extern "C" int lgf_field_footprint(const int shape[4], int precision, size_t* bytes) {
+    int64_t vol;
+    int rc = check_shape(shape, &vol);
+    if (rc) return rc;
+    if (!bytes || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
+    *bytes = size_t(3) * 72 * size_t(round_stride(vol)) * elem_size(precision);
+    return LGF_OK;
+}
+
+extern "C" int lgf_field_create(lgf_ctx* ctx, const int shape[4], int precision, lgf_field** out) {
+    if (!ctx || !out) return LGF_E_ARG;
+    *out = nullptr;
+    int64_t vol;
+    int rc = check_shape(shape, &vol);
+    if (rc) return lgf_fail(ctx, rc, "lattice extents must be positive and even");
+    if (precision != LGF_FP64 && precision != LGF_FP32) return lgf_fail(ctx, LGF_E_ARG, "unknown precision");
+    lgf_field* f = new (std::nothrow) lgf_field();
+    if (!f) return LGF_E_NOMEM;
+    f->ctx = ctx;
+    for (int i = 0; i < 4; ++i) f->shape[i] = shape[i];
+    f->prec = precision;
+    f->vol = (int)vol;
+    f->stride = round_stride(vol);
+    f->ntiles = (int)((vol + kTileSites - 1) / kTileSites);
+    const size_t bytes = size_t(72) * size_t(f->stride) * elem_size(precision);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    void* p[3] = {nullptr, nullptr, nullptr};
+    for (int i = 0; i < 3; ++i) {
+        if (hipMalloc(&p[i], bytes) != hipSuccess) {
+            for (int j = 0; j < i; ++j) (void)hipFree(p[j]);
+            delete f;
+            (void)hipGetLastError();
+            return lgf_fail(ctx, LGF_E_NOMEM, "device allocation of a field failed");
+        }
+    }
+    f->links[0] = p[0];
+    f->links[1] = p[1];
+    f->acc = p[2];
+    f->cur = 0;
+    rc = lgf_field_set_cold(f);
+    if (rc) {
+        lgf_field_destroy(f);
+        return rc;
+    }
+    *out = f;
+    return LGF_OK;
+}
+
+extern "C" int lgf_field_destroy(lgf_field* f) {
+    if (!f) return LGF_E_ARG;
+    lgf_ctx* ctx = f->ctx;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (f->solo) lgf_batch_destroy(f->solo);
+    (void)hipFree(f->links[0]);
+    (void)hipFree(f->links[1]);
+    (void)hipFree(f->acc);
+    if (f->qx) (void)hipFree(f->qx);
+    delete f;
+    return LGF_OK;
+}
+
+extern "C" int lgf_field_shape(const lgf_field* f, int shape[4], int* precision) {
+    if (!f) return LGF_E_ARG;
+    if (shape)
+        for (int i = 0; i < 4; ++i) shape[i] = f->shape[i];
+    if (precision) *precision = f->prec;
+    return LGF_OK;
+}
+
+extern "C" int lgf_field_set_cold(lgf_field* f) {
+    if (!f) return LGF_E_ARG;
+    lgf_ctx* ctx = f->ctx;
+    HIPTRY(ctx, launch_cold(f->prec, f->links[f->cur], f->stride, f->vol, ctx->stream));
+    HIPTRY(ctx, launch_cold(f->prec, f->acc, f->stride, f->vol, ctx->stream));
+    return LGF_OK;
+}
+
+// Stream `vol` sites of file-order links through the staging buffers.
+// `fill(host_chunk, s0, n)` writes n sites of file-order data starting at s0.
+template <typename Fill>
+static int upload_stream(lgf_field* f, Fill fill) {
+    lgf_ctx* ctx = f->ctx;
+    void *dev, *host;
+    int rc = ctx_staging(ctx, &dev, &host);
+    if (rc) return rc;
+    const int per = (int)(kStageBytes / (72 * sizeof(double)));
+    for (int s0 = 0; s0 < f->vol; s0 += per) {
+        const int n = std::min(per, f->vol - s0);
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));  // host staging buffer free again
+        rc = fill((double*)host, s0, n);
+        if (rc) return rc;
+        HIPTRY(ctx, hipMemcpyAsync(dev, host, size_t(n) * 72 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+        HIPTRY(ctx, launch_to_soa(f->prec, (const double*)dev, f->links[f->cur], f->stride, s0, n, ctx->stream));
+    }
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    return LGF_OK;
+}
+
+template <typename Drain>
+static int download_stream(lgf_field* f, Drain drain) {
+    lgf_ctx* ctx = f->ctx;
+    void *dev, *host;
+    int rc = ctx_staging(ctx, &dev, &host);
+    if (rc) return rc;
+    const int per = (int)(kStageBytes / (72 * sizeof(double)));
+    for (int s0 = 0; s0 < f->vol; s0 += per) {
+        const int n = std::min(per, f->vol - s0);
+        HIPTRY(ctx, launch_from_soa(f->prec, f->links[f->cur], (double*)dev, f->stride, s0, n, ctx->stream));
+        HIPTRY(ctx, hipMemcpyAsync(host, dev, size_t(n) * 72 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+        rc = drain((const double*)host, s0, n);
+        if (rc) return rc;
+    }
+    return LGF_OK;
+}
+
+extern "C" int lgf_field_upload(lgf_field* f, const double* links) {
+    if (!f || !links) return LGF_E_ARG;
+    return upload_stream(f, [&](double* h, int s0, int n) {
+        std::memcpy(h, links + size_t(s0) * 72, size_t(n) * 72 * sizeof(double));
+        return LGF_OK;
+    });
+}
+
+extern "C" int lgf_field_download(lgf_field* f, double* links) {
+    if (!f || !links) return LGF_E_ARG;
+    return download_stream(f, [&](const double* h, int s0, int n) {
+        std::memcpy(links + size_t(s0) * 72, h, size_t(n) * 72 * sizeof(double));
+        return LGF_OK;
+    });
+}
+
+// The reference's format: record (g*4+mu) of 144 bytes at byte offset
+// (g*4+mu)*144, i.e. the whole file is the file-order array.  Offsets are
+// 64-bit here (the reference computes them in int, LGF/Field/SU3_field.cpp:331).
+extern "C" int lgf_field_load_file(lgf_field* f, const char* path) {
+    if (!f || !path) return LGF_E_ARG;
+    lgf_ctx* ctx = f->ctx;
+    FILE* fp = std::fopen(path, "rb");
+    if (!fp) return lgf_fail(ctx, LGF_E_IO, std::string("cannot open configuration file ") + path);
+    std::fseek(fp, 0, SEEK_END);
+    const long long size = std::ftell(fp);
+    std::fseek(fp, 0, SEEK_SET);
+    const long long want = (long long)f->vol * 576;
+    if (size != want) {
+        std::fclose(fp);
+        return lgf_fail(ctx, LGF_E_IO, std::string("configuration file ") + path + " has " + std::to_string(size) +
+                                           " bytes, expected " + std::to_string(want));
+    }
+    int rc = upload_stream(f, [&](double* h, int, int n) {
+        size_t got = std::fread(h, sizeof(double), size_t(n) * 72, fp);
+        return got == size_t(n) * 72 ? LGF_OK : lgf_fail(ctx, LGF_E_IO, std::string("short read from ") + path);
+    });
+    std::fclose(fp);
+    return rc;
+}
+
+extern "C" int lgf_field_save_file(lgf_field* f, const char* path) {
+    if (!f || !path) return LGF_E_ARG;
+    lgf_ctx* ctx = f->ctx;
+    FILE* fp = std::fopen(path, "wb");
+    if (!fp) return lgf_fail(ctx, LGF_E_IO, std::string("cannot write configuration file ") + path);
+    int rc = download_stream(f, [&](const double* h, int, int n) {
+        size_t put = std::fwrite(h, sizeof(double), size_t(n) * 72, fp);
+        return put == size_t(n) * 72 ? LGF_OK : lgf_fail(ctx, LGF_E_IO, std::string("short write to ") + path);
+    });
+    if (std::fclose(fp) != 0 && rc == LGF_OK) rc = lgf_fail(ctx, LGF_E_IO, std::string("close failed: ") + path);
+    return rc;
+}
+
+extern "C" int lgf_field_copy(lgf_field* dst, const lgf_field* src) {
+    if (!dst || !src || dst->ctx != src->ctx) return LGF_E_ARG;
+    lgf_ctx* ctx = dst->ctx;
+    for (int i = 0; i < 4; ++i)
+        if (dst->shape[i] != src->shape[i]) return lgf_fail(ctx, LGF_E_SHAPE, "copy between different shapes");
+    HIPTRY(ctx, launch_convert(src->prec, src->links[src->cur], dst->prec, dst->links[dst->cur],
+                               int64_t(72) * dst->stride, ctx->stream));
+    return LGF_OK;
+}
+
+static int solo(lgf_field* f, lgf_batch** b) {
+    if (!f->solo) {
+        int rc = lgf_batch_create(f->ctx, &f, 1, &f->solo);
+        if (rc) return rc;
+    }
+    *b = f->solo;
+    return LGF_OK;
+}
+
+extern "C" int lgf_flow_steps(lgf_field* f, double eps, int nsteps) {
+    if (!f || nsteps < 0) return LGF_E_ARG;
+    lgf_batch* b;
+    int rc = solo(f, &b);
+    if (rc) return rc;
+    return lgf_batch_flow_steps(b, &eps, nsteps, 0);
+}
+
+extern "C" int lgf_measure(lgf_field* f, int flags, double* E, double* Q, double* Qabs, double* qraw) {
+    if (!f) return LGF_E_ARG;
+    lgf_ctx* ctx = f->ctx;
+    lgf_batch* b;
+    int rc = solo(f, &b);
+    if (rc) return rc;
+    if (qraw && !f->qx) HIPTRY(ctx, hipMalloc(&f->qx, size_t(f->vol) * sizeof(double)));
+    rc = lgf_batch_measure(b, flags, E, Q, Qabs);
+    if (rc && rc != LGF_E_NAN) return rc;
+    if (qraw) {
+        HIPTRY(ctx, hipMemcpyAsync(qraw, f->qx, size_t(f->vol) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    }
+    return rc;
+}
+
+extern "C" int lgf_diagnostics(lgf_field* f, double* plaq, double* umax, double* umean) {
+    if (!f) return LGF_E_ARG;
+    lgf_ctx* ctx = f->ctx;
+    lgf_batch* b;
+    int rc = solo(f, &b);
+    if (rc) return rc;
+    rc = batch_sync_desc(b);
+    if (rc) return rc;
+    HIPTRY(ctx, launch_diag(f->prec, b->d_desc, b->d_tiles, b->ntiles, b->ping, b->d_partials, 1, b->d_out,
+                            ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    if (plaq) *plaq = b->h_out[0] / (3.0 * 6.0 * (double)f->vol);
+    if (umean) *umean = b->h_out[1] / (4.0 * (double)f->vol);
+    if (umax) *umax = b->h_out[2];
+    return LGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// batches
+// ---------------------------------------------------------------------------
+extern "C" int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfields, lgf_batch** out) {
+    if (!ctx || !fields || nfields <= 0 || !out) return LGF_E_ARG;
+    *out = nullptr;
+    const int prec = fields[0]->prec;
+    for (int i = 0; i < nfields; ++i) {
+        if (!fields[i] || fields[i]->ctx != ctx) return lgf_fail(ctx, LGF_E_ARG, "field of another context");
+        if (fields[i]->prec != prec) return lgf_fail(ctx, LGF_E_ARG, "a batch needs one precision");
+    }
+    lgf_batch* b = new (std::nothrow) lgf_batch();
+    if (!b) return LGF_E_NOMEM;
+    b->ctx = ctx;
+    b->prec = prec;
+    b->fields.assign(fields, fields + nfields);
+    std::vector<int2> tiles;
+    b->h_desc.resize(nfields);
+    for (int i = 0; i < nfields; ++i) {
+        lgf_field* f = fields[i];
+        FieldDesc& d = b->h_desc[i];
+        std::memset(&d, 0, sizeof d);
+        d.links[0] = f->links[0];
+        d.links[1] = f->links[1];
+        d.acc = f->acc;
+        d.qx = f->qx;
+        for (int k = 0; k < 4; ++k) d.n[k] = f->shape[k];
+        d.vol = f->vol;
+        d.ntiles = f->ntiles;
+        d.stride = f->stride;
+        d.eps = 0.0;
+        d.tile0 = (int)tiles.size();
+        d.flip = f->cur;
+        for (int t = 0; t < f->ntiles; ++t) tiles.push_back(make_int2(i, t));
+    }
+    b->ntiles = (int)tiles.size();
+    b->ping = 0;
+    hipError_t e = hipMalloc(&b->d_desc, sizeof(FieldDesc) * nfields);
+    if (e == hipSuccess) e = hipMalloc(&b->d_tiles, sizeof(int2) * tiles.size());
+    if (e == hipSuccess) e = hipMalloc(&b->d_partials, sizeof(double) * 3 * tiles.size());
+    if (e == hipSuccess) e = hipMalloc(&b->d_out, sizeof(double) * 4 * nfields);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_out, sizeof(double) * 4 * nfields, hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(b->d_desc, b->h_desc.data(), sizeof(FieldDesc) * nfields, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        lgf_batch_destroy(b);
+        return lgf_hip_fail(ctx, e, "batch allocation");
+    }
+    *out = b;
+    return LGF_OK;
+}
+
+extern "C" int lgf_batch_destroy(lgf_batch* b) {
+    if (!b) return LGF_E_ARG;
+    (void)hipSetDevice(b->ctx->device);
+    (void)hipStreamSynchronize(b->ctx->stream);
+    for (int i = 0; i < 2; ++i)
+        if (b->graph[i]) (void)hipGraphExecDestroy(b->graph[i]);
+    if (b->d_desc) (void)hipFree(b->d_desc);
+    if (b->d_tiles) (void)hipFree(b->d_tiles);
+    if (b->d_partials) (void)hipFree(b->d_partials);
+    if (b->d_out) (void)hipFree(b->d_out);
+    if (b->h_out) (void)hipHostFree(b->h_out);
+    delete b;
+    return LGF_OK;
+}
+
+// Bring the device descriptor table in line with the fields (current
+// buffer, qx output) and the requested step sizes.
+int batch_sync_desc(lgf_batch* b, const double* eps) {
+    bool dirty = false;
+    for (size_t i = 0; i < b->fields.size(); ++i) {
+        lgf_field* f = b->fields[i];
+        FieldDesc& d = b->h_desc[i];
+        const int flip = f->cur ^ b->ping;
+        if (d.flip != flip || d.qx != f->qx || (eps && d.eps != eps[i])) {
+            d.flip = flip;
+            d.qx = f->qx;
+            if (eps) d.eps = eps[i];
+            dirty = true;
+        }
+    }
+    if (dirty) {
+        lgf_ctx* ctx = b->ctx;
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+        HIPTRY(ctx, hipMemcpy(b->d_desc, b->h_desc.data(), sizeof(FieldDesc) * b->h_desc.size(),
+                              hipMemcpyHostToDevice));
+    }
+    return LGF_OK;
+}
+
+static int enqueue_step(lgf_batch* b, int ping) {
+    lgf_ctx* ctx = b->ctx;
+    const int var = ctx->stage_variant[b->prec];
+    for (int stage = 0; stage < 3; ++stage) {
+        hipError_t e = launch_rk3_stage_variant(b->prec, var, stage, b->d_desc, b->d_tiles, b->ntiles, ping ^ (stage & 1),
+                                                ctx->stream);
+        if (e != hipSuccess) return lgf_hip_fail(ctx, e, "rk3 stage launch");
+    }
+    return LGF_OK;
+}
+
+static void advance(lgf_batch* b, int stages) {
+    if (stages & 1) {
+        b->ping ^= 1;
+        for (lgf_field* f : b->fields) f->cur ^= 1;
+    }
+}
+
+extern "C" int lgf_batch_flow_steps(lgf_batch* b, const double* eps, int nsteps, int use_graph) {
+    if (!b || !eps || nsteps < 0) return LGF_E_ARG;
+    lgf_ctx* ctx = b->ctx;
+    int rc = batch_sync_desc(b, eps);
+    if (rc) return rc;
+    const int var = ctx->stage_variant[b->prec];
+    while (nsteps > 0) {
+        if (use_graph && nsteps >= 2) {
+            const int g = b->ping;
+            if (b->graph[g] && b->graph_variant[g] != var) {
+                (void)hipGraphExecDestroy(b->graph[g]);
+                b->graph[g] = nullptr;
+            }
+            if (!b->graph[g]) {
+                hipGraph_t graph;
+                HIPTRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                int rc1 = enqueue_step(b, g);
+                int rc2 = enqueue_step(b, g ^ 1);
+                hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+                if (rc1 || rc2) return rc1 ? rc1 : rc2;
+                if (e != hipSuccess) return lgf_hip_fail(ctx, e, "graph capture");
+                e = hipGraphInstantiate(&b->graph[g], graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                if (e != hipSuccess) return lgf_hip_fail(ctx, e, "graph instantiate");
+                b->graph_variant[g] = var;
+            }
+            HIPTRY(ctx, hipGraphLaunch(b->graph[g], ctx->stream));
+            advance(b, 6);
+            nsteps -= 2;
+        } else {
+            rc = enqueue_step(b, b->ping);
+            if (rc) return rc;
+            advance(b, 3);
+            nsteps -= 1;
+        }
+    }
+    return LGF_OK;
+}
+
+extern "C" int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, double* Qabs) {
+    if (!b) return LGF_E_ARG;
+    lgf_ctx* ctx = b->ctx;
+    int rc = batch_sync_desc(b);
+    if (rc) return rc;
+    const int compute = (flags & LGF_MEASURE_FP64) ? LGF_FP64 : b->prec;
+    const int n = (int)b->fields.size();
+    HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, b->ntiles, b->ping, b->d_partials, ctx->stream));
+    HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials, 3, b->d_out, ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    bool finite = true;
+    for (int i = 0; i < n; ++i) {
+        const double vol = (double)b->fields[i]->vol;
+        const double e = b->h_out[4 * i + 0] * (1.0 / vol);
+        const double q = b->h_out[4 * i + 1] * (1.0 / (4.0 * kPi * kPi));
+        const double qa = b->h_out[4 * i + 2] * (1.0 / (4.0 * kPi * kPi));
+        if (E) E[i] = e;
+        if (Q) Q[i] = q;
+        if (Qabs) Qabs[i] = qa;
+        finite = finite && std::isfinite(e) && std::isfinite(q);
+    }
+    if (!finite) return lgf_fail(ctx, LGF_E_NAN, "non-finite energy density or charge");
+    return LGF_OK;
+}

@@ -1,0 +1,55 @@
+// Internal object layouts behind the opaque handles of include/lgf_flow.h.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+#include "lattice.hpp"
+
+struct lgf_batch;
+
+struct lgf_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int stage_variant[2] = {0, 0};
+    void* staging = nullptr;  // device staging chunk for uploads/downloads
+    void* pinned = nullptr;   // pinned host staging chunk
+};
+
+struct lgf_field {
+    lgf_ctx* ctx = nullptr;
+    int shape[4] = {0, 0, 0, 0};
+    int prec = 0;
+    int vol = 0;
+    int64_t stride = 0;
+    int ntiles = 0;
+    void* links[2] = {nullptr, nullptr};
+    void* acc = nullptr;
+    double* qx = nullptr;  // lazily allocated per-site charge density
+    int cur = 0;           // links[cur] holds the current configuration
+    lgf_batch* solo = nullptr;
+};
+
+struct lgf_batch {
+    lgf_ctx* ctx = nullptr;
+    int prec = 0;
+    std::vector<lgf_field*> fields;
+    std::vector<lgf::FieldDesc> h_desc;
+    lgf::FieldDesc* d_desc = nullptr;
+    int2* d_tiles = nullptr;
+    int ntiles = 0;
+    double* d_partials = nullptr;
+    double* d_out = nullptr;
+    double* h_out = nullptr;
+    int ping = 0;
+    hipGraphExec_t graph[2] = {nullptr, nullptr};  // two RK3 steps, by starting ping
+    int graph_variant[2] = {-1, -1};
+};
+
+int lgf_fail(lgf_ctx* ctx, int code, const std::string& msg);
+int lgf_hip_fail(lgf_ctx* ctx, hipError_t e, const char* what);
+int batch_sync_desc(lgf_batch* b, const double* eps = nullptr);
+int ctx_staging(lgf_ctx* ctx, void** dev, void** host);

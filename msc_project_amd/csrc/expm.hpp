@@ -1,0 +1,189 @@
+// exp(X) for X anti-Hermitian and traceless (an su(3) element), two forms.
+//
+// exp_ah_cayley: the closed Cayley-Hamilton form of Morningstar & Peardon,
+//   PRD 69 (2004) 054501, with exactly the branch structure of the reference
+//   HermTrLessExp (LGF/Math/SU3_mat.cpp:216-289): series for xi0 when
+//   |w| <= 0.05, Taylor coefficients when c1 <= 1e-4, and the c0 < 0
+//   conjugation symmetry.  Used by the FP64 parity path, where results must
+//   follow the reference's formula to 1e-12 over hundreds of steps.
+//
+// exp_ah_series: the same f0 + f1 Q + f2 Q^2 decomposition, but with the f_j
+//   summed from the exponential series using the Cayley-Hamilton recurrence
+//   Q^{n+1} = c0 d_n + (a_n + c1 d_n) Q + b_n Q^2 for Q^n = a_n + b_n Q + d_n Q^2.
+//   No transcendental functions; NTERMS = 13 gives a truncation error below
+//   |lambda|^13/13! (< 2e-10 for |lambda| <= 1, where lambda are the
+//   eigenvalues of Q).  Links whose exponent is larger than that fall back to
+//   the closed form in double precision.  Used by the mixed-precision path.
+#pragma once
+#include "su3.hpp"
+
+namespace lgf {
+
+struct Cx {
+    double re, im;
+};
+
+// Closed form, double precision, reference branch structure.
+LGF_DEV M3<double> exp_ah_cayley(const M3<double>& X) {
+    M3<double> Q;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {  // Q = -i X
+        Q.v[2 * i] = X.v[2 * i + 1];
+        Q.v[2 * i + 1] = -X.v[2 * i];
+    }
+    M3<double> Q2 = mul_nn(Q, Q);
+    double c0 = retr_mul(Q, Q2) / 3.0;
+    double c1 = retr(Q2) / 2.0;
+    double c1_sqrt = sqrt(c1);
+    double ratio = fabs(c0) / (2.0 * pow((c1 / 3.0), 1.5));
+    double theta3 = acos(ratio) / 3.0;
+    double u = sqrt(1.0 / 3.0) * c1_sqrt * cos(theta3);
+    double u2 = u * u;
+    double ucos = cos(u), usin = sin(u);
+    double w = c1_sqrt * sin(theta3);
+    double w2 = w * w;
+    double wcos = cos(w);
+    double xi0 = (fabs(w) > 0.05) ? sin(w) / w : 1.0 - w2 * (1.0 - w2 * (1 - w2 / 42.0) / 20.0) / 6.0;
+    Cx f0, f1, f2;
+    if (c1 > 0.0001) {
+        double div = 1.0 / (9.0 * u2 - w2);
+        Cx e2 = {(2.0 * ucos * ucos - 1.0), 2.0 * ucos * usin};
+        Cx em = {ucos, -usin};
+        Cx h = {8.0 * u2 * wcos, 2.0 * u * (3.0 * u2 + w2) * xi0};
+        double s = u2 - w2;
+        f0.re = div * (s * e2.re + (em.re * h.re - em.im * h.im));
+        f0.im = div * (s * e2.im + (em.re * h.im + em.im * h.re));
+        h = Cx{2.0 * u * wcos, (w2 - 3.0 * u2) * xi0};
+        double t = 2.0 * u;
+        f1.re = div * (t * e2.re - (em.re * h.re - em.im * h.im));
+        f1.im = div * (t * e2.im - (em.re * h.im + em.im * h.re));
+        h = Cx{wcos, 3.0 * u * xi0};
+        f2.re = div * (e2.re - (em.re * h.re - em.im * h.im));
+        f2.im = div * (e2.im - (em.re * h.im + em.im * h.re));
+        if (c0 < 0.0) {
+            f0.im = -f0.im;
+            f1.re = -f1.re;
+            f2.im = -f2.im;
+        }
+    } else {
+        double c0s = c0 * c0;
+        f0 = Cx{1.0 - c0s / 720.0, -c0 * (1.0 - c1 * (1 - c1 / 42.0) / 20.0) / 6.0};
+        f1 = Cx{c0 * (1.0 - c1 * (1.0 - 3.0 * c1 / 112.0) / 15.0) / 24.0,
+                1.0 - c1 * (1.0 - c1 * (1 - c1 / 42.0) / 20.0) / 6.0 - c0s / 5040.0};
+        f2 = Cx{(-1.0 + c1 * (1.0 - c1 * (1.0 - c1 / 56.0) / 30.0) / 12.0 + c0s / 20160.0) / 2.0,
+                (c0 * (1.0 - c1 * (1.0 - c1 / 48.0) / 21.0) / 60.0) / 2.0};
+    }
+    M3<double> o;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        double qr = Q.v[2 * i], qi = Q.v[2 * i + 1], pr = Q2.v[2 * i], pi = Q2.v[2 * i + 1];
+        o.v[2 * i] = (qr * f1.re - qi * f1.im) + (pr * f2.re - pi * f2.im);
+        o.v[2 * i + 1] = (qr * f1.im + qi * f1.re) + (pr * f2.im + pi * f2.re);
+    }
+    o.v[0] += f0.re;
+    o.v[1] += f0.im;
+    o.v[8] += f0.re;
+    o.v[9] += f0.im;
+    o.v[16] += f0.re;
+    o.v[17] += f0.im;
+    return o;
+}
+
+// Power series through the Cayley-Hamilton recurrence, real type R.
+template <typename R, int NTERMS = 13>
+LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
+    M3<R> Q;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Q.v[2 * i] = X.v[2 * i + 1];
+        Q.v[2 * i + 1] = -X.v[2 * i];
+    }
+    M3<R> Q2 = mul_nn(Q, Q);
+    R c0 = retr_mul(Q, Q2) * (R(1) / R(3));
+    R c1 = retr(Q2) * R(0.5);
+    if (c1 > R(0.75)) {  // |lambda| may exceed 1: closed form in double
+        M3<double> Xd;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Xd.v[i] = (double)X.v[i];
+        M3<double> Ed = exp_ah_cayley(Xd);
+        M3<R> E;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = (R)Ed.v[i];
+        return E;
+    }
+    // f_j = sum_n i^n/n! * coef_j(Q^n); real part from n = 0,2 (mod 4) with
+    // signs +,-, imaginary part from n = 1,3 (mod 4) with signs +,-.
+    R a = R(1), b = R(0), d = R(0);
+    R f0r = R(1), f0i = R(0), f1r = R(0), f1i = R(0), f2r = R(0), f2i = R(0);
+    R fact = R(1);
+#pragma unroll
+    for (int n = 1; n < NTERMS; ++n) {
+        R an = c0 * d, bn = a + c1 * d, dn = b;
+        a = an;
+        b = bn;
+        d = dn;
+        fact = fact / R(n);  // folded at compile time
+        switch (n & 3) {
+            case 0: f0r += fact * a; f1r += fact * b; f2r += fact * d; break;
+            case 1: f0i += fact * a; f1i += fact * b; f2i += fact * d; break;
+            case 2: f0r -= fact * a; f1r -= fact * b; f2r -= fact * d; break;
+            default: f0i -= fact * a; f1i -= fact * b; f2i -= fact * d; break;
+        }
+    }
+    M3<R> o;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        R qr = Q.v[2 * i], qi = Q.v[2 * i + 1], pr = Q2.v[2 * i], pi = Q2.v[2 * i + 1];
+        o.v[2 * i] = (qr * f1r - qi * f1i) + (pr * f2r - pi * f2i);
+        o.v[2 * i + 1] = (qr * f1i + qi * f1r) + (pr * f2i + pi * f2r);
+    }
+    o.v[0] += f0r;
+    o.v[1] += f0i;
+    o.v[8] += f0r;
+    o.v[9] += f0i;
+    o.v[16] += f0r;
+    o.v[17] += f0i;
+    return o;
+}
+
+// Project onto SU(3) in double precision: Gram-Schmidt on rows 0 and 1,
+// row 2 = conj(row0 x row1).
+LGF_DEV void reunitarize(M3<double>& m) {
+    double n = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) n += m.v[i] * m.v[i];
+    n = 1.0 / sqrt(n);
+#pragma unroll
+    for (int i = 0; i < 6; ++i) m.v[i] *= n;
+    double pr = 0, pi = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double ar = m.v[2 * c], ai = m.v[2 * c + 1], br = m.v[6 + 2 * c], bi = m.v[7 + 2 * c];
+        pr += ar * br + ai * bi;
+        pi += ar * bi - ai * br;
+    }
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        double ar = m.v[2 * c], ai = m.v[2 * c + 1];
+        m.v[6 + 2 * c] -= pr * ar - pi * ai;
+        m.v[7 + 2 * c] -= pr * ai + pi * ar;
+    }
+    n = 0;
+#pragma unroll
+    for (int i = 6; i < 12; ++i) n += m.v[i] * m.v[i];
+    n = 1.0 / sqrt(n);
+#pragma unroll
+    for (int i = 6; i < 12; ++i) m.v[i] *= n;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        double xr = (m.v[2 * c1] * m.v[6 + 2 * c2] - m.v[2 * c1 + 1] * m.v[7 + 2 * c2]) -
+                    (m.v[2 * c2] * m.v[6 + 2 * c1] - m.v[2 * c2 + 1] * m.v[7 + 2 * c1]);
+        double xi = (m.v[2 * c1] * m.v[7 + 2 * c2] + m.v[2 * c1 + 1] * m.v[6 + 2 * c2]) -
+                    (m.v[2 * c2] * m.v[7 + 2 * c1] + m.v[2 * c2 + 1] * m.v[6 + 2 * c1]);
+        m.v[12 + 2 * c] = xr;
+        m.v[13 + 2 * c] = -xi;
+    }
+}
+
+}  // namespace lgf

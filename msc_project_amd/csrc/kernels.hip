@@ -1,0 +1,554 @@
+// gfx950 kernels of the Wilson-flow / clover-observable path.
+//
+// k_rk3_stage  one fused Luescher RK3 stage for every link of every field in
+//              a batch: staple sum, traceless anti-Hermitian projection
+//              (flow gradient Z), low-storage accumulator update, SU(3)
+//              exponential and left-multiplication into the other ping-pong
+//              buffer.  Follows LGF/Gradient_Flow/GradientFlow.cpp:22-60,
+//              LGF/Action/Gauge/Wilson.cpp:55-71, LGF/Field/SU3_field.cpp:519-530.
+//              The reference keeps Z0, Z1, Z2; with X0 = Z0,
+//              X1 = 8/9 Z1 - 17/36 X0 the three exponents are 1/4 eps X0,
+//              eps X1 and eps (3/4 Z2 - X1), which is the same integrator
+//              with one accumulator (also the form of the thesis algorithm,
+//              doc/thesis.tex:2569-2600).
+// k_clover     1x1 clover F (energy density) and the improved F (19/9 1x1 +
+//              1/36 2x2 - 32/45 1x2 + 1/15 1x3, LGF/Field/SU3_field.cpp:533-797)
+//              for the plane pairs of the topological charge; per-site q(x),
+//              per-tile FP64 partial sums.
+// k_reduce     deterministic per-field FP64 sums of the tile partials.
+// plus layout conversion (file order <-> SoA), cold start, plaquette and
+// unitarity diagnostics.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "expm.hpp"
+#include "kernels.hpp"
+#include "lattice.hpp"
+#include "su3.hpp"
+
+namespace lgf {
+
+// ---------------------------------------------------------------------------
+// RK3 stage
+// ---------------------------------------------------------------------------
+template <typename S, typename C, int STAGE, int EXPMODE, int WPE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage(const FieldDesc* __restrict__ fds,
+                                                     const int2* __restrict__ tiles, int ping) {
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int2 tm = tiles[t];
+    const FieldDesc& d = fds[tm.x];
+    const int s = tm.y * kTileSites + threadIdx.x;
+    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (s >= d.vol) return;
+    const int cur = ping ^ d.flip;
+    const S* __restrict__ U = (const S*)d.links[cur];
+    S* __restrict__ Un = (S*)d.links[cur ^ 1];
+    S* __restrict__ Xa = (S*)d.acc;
+    const int64_t st = d.stride;
+    const Geo g = make_geo(d, s);
+    const int fmu = nb_off(g, mu, +1);
+
+    // A_mu(x) = sum_{nu != mu} U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
+    //                        + (U_mu(x-nu) U_nu(x-nu+mu))^dag U_nu(x-nu)
+    M3<C> A;
+    set_zero(A);
+#pragma unroll 1
+    for (int k = 0; k < 3; ++k) {
+        const int nu = k + (k >= mu ? 1 : 0);
+        const int fnu = nb_off(g, nu, +1), bnu = nb_off(g, nu, -1);
+        {
+            M3<C> a = load_link<S, C>(U, st, nu, s);
+            M3<C> b = load_link<S, C>(U, st, mu, s + fnu);
+            M3<C> tt = mul_nn(a, b);
+            M3<C> c = load_link<S, C>(U, st, nu, s + fmu);
+            add_to(A, mul_nd(c, tt));
+        }
+        {
+            M3<C> a = load_link<S, C>(U, st, mu, s + bnu);
+            M3<C> b = load_link<S, C>(U, st, nu, s + bnu + fmu);
+            M3<C> tt = mul_nn(a, b);
+            M3<C> c = load_link<S, C>(U, st, nu, s + bnu);
+            add_to(A, mul_dn(tt, c));
+        }
+    }
+    const M3<C> Umu = load_link<S, C>(U, st, mu, s);
+    const M3<C> Z = ta_scaled(mul_nn(Umu, A), C(-0.5));
+
+    const C eps = (C)d.eps;
+    M3<C> E;
+    if (STAGE == 0) {
+        store_link<S, C>(Xa, st, mu, s, Z);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
+    } else if (STAGE == 1) {
+        M3<C> X0 = load_link<S, C>(Xa, st, mu, s);
+        M3<C> X1;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
+        store_link<S, C>(Xa, st, mu, s, X1);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
+    } else {
+        M3<C> X1 = load_link<S, C>(Xa, st, mu, s);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
+    }
+
+    M3<C> ex;
+    if constexpr (EXPMODE == 0) {
+        ex = exp_ah_cayley(E);  // C == double
+    } else {
+        ex = exp_ah_series<C>(E);
+    }
+    M3<C> Unew = mul_nn(ex, Umu);
+    if constexpr (EXPMODE == 1 && STAGE == 2) {
+        // once per RK3 step: project back onto SU(3) in double precision
+        M3<double> Ud;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
+        reunitarize(Ud);
+        store_link<S, double>(Un, st, mu, s, Ud);
+    } else {
+        store_link<S, C>(Un, st, mu, s, Unew);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// clovers, E, Q, q(x)
+// ---------------------------------------------------------------------------
+// Path products.  A step along +d multiplies by U_d(here) and moves to
+// here+d; a step along -d moves to here-d and multiplies by U_d(here)^dag.
+// The first link of a path initialises the product.
+template <typename S, typename C>
+LGF_DEV void path_first(const S* __restrict__ U, int64_t st, Geo& g, M3<C>& P, int dir, int sgn) {
+    if (sgn > 0) {
+        P = load_link<S, C>(U, st, dir, g.s);
+        geo_move(g, dir, +1);
+    } else {
+        geo_move(g, dir, -1);
+        const M3<C> L = load_link<S, C>(U, st, dir, g.s);
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                P.v[2 * (3 * r + c)] = L.v[2 * (3 * c + r)];
+                P.v[2 * (3 * r + c) + 1] = -L.v[2 * (3 * c + r) + 1];
+            }
+    }
+}
+
+template <typename S, typename C>
+LGF_DEV void path_step(const S* __restrict__ U, int64_t st, Geo& g, M3<C>& P, int dir, int sgn) {
+    if (sgn > 0) {
+        const M3<C> L = load_link<S, C>(U, st, dir, g.s);
+        P = mul_nn(P, L);
+        geo_move(g, dir, +1);
+    } else {
+        geo_move(g, dir, -1);
+        const M3<C> L = load_link<S, C>(U, st, dir, g.s);
+        P = mul_nd(P, L);
+    }
+}
+
+// Sum of the four leaves of extent (lmu x lnu) in the (mu,nu) plane,
+// weighted by w, added to acc (and, unweighted, to acc2 when ALSO_PLAIN).
+// The leaves are oriented as the reference's (first leg along a, second
+// along b, (a,b) in {(+nu,+mu), (+mu,-nu), (-mu,+nu), (-nu,-mu)}), e.g. the
+// first 1x1 leaf is U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag U_mu(x)^dag
+// (LGF/Field/SU3_field.cpp:550-560); the 2x2, 1x2 and 1x3 clovers of
+// LGF/Field/SU3_field.cpp:598-792 are the same four leaves with longer legs.
+template <typename S, typename C, bool ALSO_PLAIN>
+LGF_DEV void add_leaves(const S* __restrict__ U, int64_t st, const Geo& g0, int mu, int nu, int lmu,
+                        int lnu, C w, M3<C>& acc, M3<C>& acc2) {
+#pragma unroll 1
+    for (int q = 0; q < 4; ++q) {
+        const int adir = (q == 0 || q == 3) ? nu : mu;
+        const int asg = (q == 2 || q == 3) ? -1 : +1;
+        const int bdir = (q == 0 || q == 3) ? mu : nu;
+        const int bsg = (q == 1 || q == 3) ? -1 : +1;
+        const int la = (adir == mu) ? lmu : lnu;
+        const int lb = (bdir == mu) ? lmu : lnu;
+        Geo g = g0;
+        M3<C> P;
+        path_first<S, C>(U, st, g, P, adir, asg);
+#pragma unroll 1
+        for (int i = 1; i < la; ++i) path_step<S, C>(U, st, g, P, adir, asg);
+#pragma unroll 1
+        for (int i = 0; i < lb; ++i) path_step<S, C>(U, st, g, P, bdir, bsg);
+#pragma unroll 1
+        for (int i = 0; i < la; ++i) path_step<S, C>(U, st, g, P, adir, -asg);
+#pragma unroll 1
+        for (int i = 0; i < lb; ++i) path_step<S, C>(U, st, g, P, bdir, -bsg);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) acc.v[i] += w * P.v[i];
+        if constexpr (ALSO_PLAIN) {
+#pragma unroll
+            for (int i = 0; i < 18; ++i) acc2.v[i] += P.v[i];
+        }
+    }
+}
+
+// F = 1/2 (-i) [ (C - C^dag) - 1/3 Tr(C - C^dag) ] (LGF/Field/SU3_field.cpp:575-579)
+template <typename C>
+LGF_DEV M3<C> project_F(const M3<C>& Cl) {
+    M3<C> T = ta_scaled(Cl, C(1));
+    M3<C> F;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        F.v[2 * i] = C(0.5) * T.v[2 * i + 1];
+        F.v[2 * i + 1] = C(-0.5) * T.v[2 * i];
+    }
+    return F;
+}
+
+// improved F of plane (mu,nu) and the plain 1x1 F, both projected
+template <typename S, typename C>
+LGF_DEV void plane_F(const S* __restrict__ U, int64_t st, const Geo& g, int mu, int nu, M3<C>& Fimp,
+                     M3<C>& F11) {
+    M3<C> Ci, C11;
+    set_zero(Ci);
+    set_zero(C11);
+    const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
+    add_leaves<S, C, true>(U, st, g, mu, nu, 1, 1, C(0.25) * k1, Ci, C11);
+    add_leaves<S, C, false>(U, st, g, mu, nu, 2, 2, C(0.25) * k2, Ci, C11);
+    add_leaves<S, C, false>(U, st, g, mu, nu, 1, 2, C(0.125) * k3, Ci, C11);
+    add_leaves<S, C, false>(U, st, g, mu, nu, 2, 1, C(0.125) * k3, Ci, C11);
+    add_leaves<S, C, false>(U, st, g, mu, nu, 1, 3, C(0.125) * k4, Ci, C11);
+    add_leaves<S, C, false>(U, st, g, mu, nu, 3, 1, C(0.125) * k4, Ci, C11);
+#pragma unroll
+    for (int i = 0; i < 18; ++i) C11.v[i] *= C(0.25);
+    Fimp = project_F(Ci);
+    F11 = project_F(C11);
+}
+
+// Re Tr(A B) for Hermitian A, B given as 9 reals each:
+// (A00, A11, A22, Re A01, Im A01, Re A02, Im A02, Re A12, Im A12)
+LGF_DEV double herm_dot(const double* a, const double* b) {
+    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] +
+           2.0 * (a[3] * b[3] + a[4] * b[4] + a[5] * b[5] + a[6] * b[6] + a[7] * b[7] + a[8] * b[8]);
+}
+
+template <typename C>
+LGF_DEV void herm_pack(const M3<C>& F, double* h) {
+    h[0] = (double)F.v[0];
+    h[1] = (double)F.v[8];
+    h[2] = (double)F.v[16];
+    h[3] = (double)F.v[2];
+    h[4] = (double)F.v[3];
+    h[5] = (double)F.v[4];
+    h[6] = (double)F.v[5];
+    h[7] = (double)F.v[10];
+    h[8] = (double)F.v[11];
+}
+
+// One thread per (site, plane).  The six planes are taken in the order of
+// the reference's charge, (3,2),(0,1) | (3,1),(2,0) | (3,0),(1,2)
+// (LGF/Observables/TopologicalCharge.cpp:12-16); they visit each plane
+// once, which the energy density sums over (EnergyDensity.cpp:13-20).
+// Per site: e(x) = sum_planes Re Tr(F11 F11), q(x) = sum_pairs Re Tr(Fa Fb)
+// with the improved F; tiles write (sum e, sum q, sum |q|) partials and,
+// when requested, q(x).
+template <typename S, typename C>
+__global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clover(
+    const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int ping, double* __restrict__ partials) {
+    __shared__ double sF[6][9][kTileSites];
+    __shared__ double sE[6][kTileSites];
+    const int tix = xcd_remap(blockIdx.x, gridDim.x);
+    const int2 tm = tiles[tix];
+    const FieldDesc& d = fds[tm.x];
+    const int s = tm.y * kTileSites + threadIdx.x;
+    const int p = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const bool live = s < d.vol;
+    double e = 0.0;
+    double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (live) {
+        const S* __restrict__ U = (const S*)d.links[ping ^ d.flip];
+        const Geo g = make_geo(d, s);
+        // plane p: (3,2),(0,1),(3,1),(2,0),(3,0),(1,2)
+        const int mu = (p & 1) ? (p == 1 ? 0 : (p == 3 ? 2 : 1)) : 3;
+        const int nu = (p & 1) ? (p == 1 ? 1 : (p == 3 ? 0 : 2)) : (p == 0 ? 2 : (p == 2 ? 1 : 0));
+        M3<C> Fi, F11;
+        plane_F<S, C>(U, d.stride, g, mu, nu, Fi, F11);
+        e = (double)retr_mul(F11, F11);
+        herm_pack(Fi, h);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) sF[p][i][threadIdx.x] = h[i];
+    sE[p][threadIdx.x] = e;
+    __syncthreads();
+    if (p == 0) {
+        double a[9], b[9], qs = 0.0, es = 0.0;
+#pragma unroll 1
+        for (int k = 0; k < 3; ++k) {
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                a[i] = sF[2 * k][i][threadIdx.x];
+                b[i] = sF[2 * k + 1][i][threadIdx.x];
+            }
+            qs += herm_dot(a, b);
+        }
+#pragma unroll
+        for (int k = 0; k < 6; ++k) es += sE[k][threadIdx.x];
+        if (live && d.qx) d.qx[s] = qs;
+        double qa = fabs(qs);
+        // wavefront reduction (64 lanes), fixed order
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            es += __shfl_down(es, off, 64);
+            qs += __shfl_down(qs, off, 64);
+            qa += __shfl_down(qa, off, 64);
+        }
+        if (threadIdx.x == 0) {
+            const int gt = d.tile0 + tm.y;
+            partials[3 * gt + 0] = es;
+            partials[3 * gt + 1] = qs;
+            partials[3 * gt + 2] = qa;
+        }
+    }
+}
+
+// One workgroup per field: sum that field's tile partials (records of
+// `pstride` doubles, components 0..ncomp-1) in a fixed order, so that a
+// field's result does not depend on the other fields of its batch.
+// out[4*f + c]; for the clover partials c = {sum e, sum q, sum |q|}.
+__global__ __launch_bounds__(256) void k_reduce(const FieldDesc* __restrict__ fds, const double* __restrict__ partials,
+                                                int pstride, int ncomp, double* __restrict__ out) {
+    __shared__ double red[256];
+    const FieldDesc& d = fds[blockIdx.x];
+    for (int c = 0; c < ncomp; ++c) {
+        double acc = 0.0;
+        for (int t = threadIdx.x; t < d.ntiles; t += 256) acc += partials[pstride * (d.tile0 + t) + c];
+        red[threadIdx.x] = acc;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[4 * blockIdx.x + c] = red[0];
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+// average plaquette (LGF/Field/SU3_field.cpp:485-499) partials: sum over the
+// six planes of Re Tr U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag; and
+// unitarity deviation max / sum of ||U^dag U - 1||_F over the 4 links.
+template <typename S>
+__global__ __launch_bounds__(64) void k_diag(const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles,
+                                             int ping, double* __restrict__ partials) {
+    const int tix = xcd_remap(blockIdx.x, gridDim.x);
+    const int2 tm = tiles[tix];
+    const FieldDesc& d = fds[tm.x];
+    const int s = tm.y * kTileSites + threadIdx.x;
+    double pl = 0.0, umax = 0.0, usum = 0.0;
+    if (s < d.vol) {
+        const S* __restrict__ U = (const S*)d.links[ping ^ d.flip];
+        const Geo g = make_geo(d, s);
+        for (int mu = 0; mu < 4; ++mu) {
+            M3<double> Um = load_link<S, double>(U, d.stride, mu, s);
+            M3<double> P = mul_dn(Um, Um);
+            P.v[0] -= 1.0;
+            P.v[8] -= 1.0;
+            P.v[16] -= 1.0;
+            double n = 0.0;
+#pragma unroll
+            for (int i = 0; i < 18; ++i) n += P.v[i] * P.v[i];
+            n = sqrt(n);
+            umax = fmax(umax, n);
+            usum += n;
+            for (int nu = mu + 1; nu < 4; ++nu) {
+                M3<double> a = mul_nn(Um, load_link<S, double>(U, d.stride, nu, s + nb_off(g, mu, +1)));
+                M3<double> b = mul_nn(load_link<S, double>(U, d.stride, nu, s),
+                                      load_link<S, double>(U, d.stride, mu, s + nb_off(g, nu, +1)));
+                pl += retr(mul_nd(a, b));
+            }
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        pl += __shfl_down(pl, off, 64);
+        usum += __shfl_down(usum, off, 64);
+        umax = fmax(umax, __shfl_down(umax, off, 64));
+    }
+    if (threadIdx.x == 0) {
+        const int gt = d.tile0 + tm.y;
+        partials[3 * gt + 0] = pl;
+        partials[3 * gt + 1] = usum;
+        partials[3 * gt + 2] = umax;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_reduce_max(const FieldDesc* __restrict__ fds,
+                                                    const double* __restrict__ partials, int ncomp, int comp,
+                                                    double* __restrict__ out) {
+    __shared__ double red[256];
+    const FieldDesc& d = fds[blockIdx.x];
+    double acc = 0.0;
+    for (int t = threadIdx.x; t < d.ntiles; t += 256) acc = fmax(acc, partials[ncomp * (d.tile0 + t) + comp]);
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if ((int)threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[4 * blockIdx.x + comp] = red[0];
+}
+
+// ---------------------------------------------------------------------------
+// layout conversion and initialisation
+// ---------------------------------------------------------------------------
+// file order (site-major, 4 links x 18 doubles) for sites [s0, s0+n) -> SoA
+template <typename S>
+__global__ __launch_bounds__(256) void k_fileorder_to_soa(const double* __restrict__ src, S* __restrict__ dst,
+                                                          int64_t stride, int s0, int n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // element within chunk
+    if (i >= (int64_t)n * 72) return;
+    const int ls = (int)(i / 72), r = (int)(i % 72);
+    const int mu = r / 18, c = r % 18;
+    dst[((int64_t)mu * 18 + c) * stride + s0 + ls] = (S)src[i];
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_soa_to_fileorder(const S* __restrict__ src, double* __restrict__ dst,
+                                                          int64_t stride, int s0, int n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)n * 72) return;
+    const int ls = (int)(i / 72), r = (int)(i % 72);
+    const int mu = r / 18, c = r % 18;
+    dst[i] = (double)src[((int64_t)mu * 18 + c) * stride + s0 + ls];
+}
+
+template <typename SI, typename SO>
+__global__ __launch_bounds__(256) void k_convert(const SI* __restrict__ src, SO* __restrict__ dst, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (SO)src[i];
+}
+
+template <typename S>
+__global__ __launch_bounds__(256) void k_cold(S* __restrict__ dst, int64_t stride, int vol) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (int64_t)72 * stride) return;
+    const int c = (int)((i / stride) % 18);
+    const int64_t s = i % stride;
+    (void)vol;
+    dst[i] = (S)((c == 0 || c == 8 || c == 16) && s < vol ? 1.0 : 0.0);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+template <typename S, typename C, int EXPMODE, int WPE>
+static hipError_t stage_launch(int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
+                               hipStream_t stream) {
+    dim3 grid(ntiles), block(kTileSites, 4);
+    if (stage == 0)
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 0, EXPMODE, WPE>), grid, block, 0, stream, fds, tiles, ping);
+    else if (stage == 1)
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 1, EXPMODE, WPE>), grid, block, 0, stream, fds, tiles, ping);
+    else
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 2, EXPMODE, WPE>), grid, block, 0, stream, fds, tiles, ping);
+    return hipGetLastError();
+}
+
+// Stage-kernel variants differ in the register budget (minimum waves per
+// SIMD) the compiler is held to; variant 0 is the default.
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 3 : 4; }
+
+hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
+                                    int ntiles, int ping, hipStream_t stream) {
+    if (prec == kPrecFP64) {
+        switch (variant) {
+            case 1: return stage_launch<double, double, 0, 2>(stage, fds, tiles, ntiles, ping, stream);
+            case 2: return stage_launch<double, double, 0, 4>(stage, fds, tiles, ntiles, ping, stream);
+            default: return stage_launch<double, double, 0, 3>(stage, fds, tiles, ntiles, ping, stream);
+        }
+    }
+    switch (variant) {
+        case 1: return stage_launch<float, float, 1, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 2: return stage_launch<float, float, 1, 5>(stage, fds, tiles, ntiles, ping, stream);
+        case 3: return stage_launch<float, float, 1, 2>(stage, fds, tiles, ntiles, ping, stream);
+        default: return stage_launch<float, float, 1, 3>(stage, fds, tiles, ntiles, ping, stream);
+    }
+}
+
+hipError_t launch_rk3_stage(int prec, int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
+                            hipStream_t stream) {
+    return launch_rk3_stage_variant(prec, 0, stage, fds, tiles, ntiles, ping, stream);
+}
+
+hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
+                         double* partials, hipStream_t stream) {
+    dim3 grid(ntiles), block(kTileSites, 6);
+    if (prec == kPrecFP64)
+        hipLaunchKernelGGL((k_clover<double, double>), grid, block, 0, stream, fds, tiles, ping, partials);
+    else if (compute == kPrecFP64)
+        hipLaunchKernelGGL((k_clover<float, double>), grid, block, 0, stream, fds, tiles, ping, partials);
+    else
+        hipLaunchKernelGGL((k_clover<float, float>), grid, block, 0, stream, fds, tiles, ping, partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int ncomp, double* out,
+                         hipStream_t stream) {
+    hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(256), 0, stream, fds, partials, 3, ncomp, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int ntiles, int ping, double* partials,
+                       int nfields, double* out, hipStream_t stream) {
+    if (prec == kPrecFP64)
+        hipLaunchKernelGGL((k_diag<double>), dim3(ntiles), dim3(64), 0, stream, fds, tiles, ping, partials);
+    else
+        hipLaunchKernelGGL((k_diag<float>), dim3(ntiles), dim3(64), 0, stream, fds, tiles, ping, partials);
+    hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(256), 0, stream, fds, partials, 3, 2, out);
+    hipLaunchKernelGGL(k_reduce_max, dim3(nfields), dim3(256), 0, stream, fds, partials, 3, 2, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_to_soa(int prec, const double* src, void* dst, int64_t stride, int s0, int n, hipStream_t stream) {
+    int64_t total = (int64_t)n * 72;
+    dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    if (prec == kPrecFP64)
+        hipLaunchKernelGGL(k_fileorder_to_soa<double>, grid, block, 0, stream, src, (double*)dst, stride, s0, n);
+    else
+        hipLaunchKernelGGL(k_fileorder_to_soa<float>, grid, block, 0, stream, src, (float*)dst, stride, s0, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_from_soa(int prec, const void* src, double* dst, int64_t stride, int s0, int n,
+                           hipStream_t stream) {
+    int64_t total = (int64_t)n * 72;
+    dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    if (prec == kPrecFP64)
+        hipLaunchKernelGGL(k_soa_to_fileorder<double>, grid, block, 0, stream, (const double*)src, dst, stride, s0, n);
+    else
+        hipLaunchKernelGGL(k_soa_to_fileorder<float>, grid, block, 0, stream, (const float*)src, dst, stride, s0, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_convert(int prec_in, const void* src, int prec_out, void* dst, int64_t n, hipStream_t stream) {
+    dim3 grid((unsigned)((n + 255) / 256)), block(256);
+    if (prec_in == kPrecFP64 && prec_out == kPrecFP64)
+        hipLaunchKernelGGL((k_convert<double, double>), grid, block, 0, stream, (const double*)src, (double*)dst, n);
+    else if (prec_in == kPrecFP64)
+        hipLaunchKernelGGL((k_convert<double, float>), grid, block, 0, stream, (const double*)src, (float*)dst, n);
+    else if (prec_out == kPrecFP64)
+        hipLaunchKernelGGL((k_convert<float, double>), grid, block, 0, stream, (const float*)src, (double*)dst, n);
+    else
+        hipLaunchKernelGGL((k_convert<float, float>), grid, block, 0, stream, (const float*)src, (float*)dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_cold(int prec, void* dst, int64_t stride, int vol, hipStream_t stream) {
+    int64_t total = (int64_t)72 * stride;
+    dim3 grid((unsigned)((total + 255) / 256)), block(256);
+    if (prec == kPrecFP64)
+        hipLaunchKernelGGL(k_cold<double>, grid, block, 0, stream, (double*)dst, stride, vol);
+    else
+        hipLaunchKernelGGL(k_cold<float>, grid, block, 0, stream, (float*)dst, stride, vol);
+    return hipGetLastError();
+}
+
+}  // namespace lgf

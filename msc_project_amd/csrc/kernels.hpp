@@ -1,0 +1,30 @@
+// Host-callable launchers of the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lattice.hpp"
+
+namespace lgf {
+
+constexpr int kPrecFP64 = 0;  // FP64 links, accumulator and arithmetic
+constexpr int kPrecFP32 = 1;  // FP32 links/arithmetic, FP64 reductions and reunitarisation
+
+int num_stage_variants(int prec);
+hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
+                                    int ntiles, int ping, hipStream_t stream);
+hipError_t launch_rk3_stage(int prec, int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
+                            hipStream_t stream);
+hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
+                         double* partials, hipStream_t stream);
+hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int ncomp, double* out,
+                         hipStream_t stream);
+hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int ntiles, int ping, double* partials,
+                       int nfields, double* out, hipStream_t stream);
+hipError_t launch_to_soa(int prec, const double* src, void* dst, int64_t stride, int s0, int n, hipStream_t stream);
+hipError_t launch_from_soa(int prec, const void* src, double* dst, int64_t stride, int s0, int n,
+                           hipStream_t stream);
+hipError_t launch_convert(int prec_in, const void* src, int prec_out, void* dst, int64_t n, hipStream_t stream);
+hipError_t launch_cold(int prec, void* dst, int64_t stride, int vol, hipStream_t stream);
+
+}  // namespace lgf

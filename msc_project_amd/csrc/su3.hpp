@@ -1,0 +1,149 @@
+// 3x3 complex matrix arithmetic for gfx950 vector ALUs.
+//
+// A matrix lives in 18 registers: element (r,c) is (v[2*(3r+c)], v[2*(3r+c)+1])
+// = (Re, Im), row-major -- the element order of the reference's su3_mat
+// (LGF/Math/SU3_mat.h:15-17) and of its saved-configuration records.
+// Everything is templated on the real type R (double for the parity path,
+// float for the mixed-precision path) and force-inlined so that the compiler
+// keeps matrices in VGPRs and schedules the products as FMA chains.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#define LGF_DEV __device__ __forceinline__
+
+namespace lgf {
+
+template <typename R>
+struct M3 {
+    R v[18];
+};
+
+template <typename R>
+LGF_DEV R& re(M3<R>& m, int r, int c) { return m.v[2 * (3 * r + c)]; }
+template <typename R>
+LGF_DEV R& im(M3<R>& m, int r, int c) { return m.v[2 * (3 * r + c) + 1]; }
+
+template <typename R>
+LGF_DEV void set_zero(M3<R>& m) {
+#pragma unroll
+    for (int i = 0; i < 18; ++i) m.v[i] = R(0);
+}
+
+template <typename R>
+LGF_DEV void set_unit(M3<R>& m) {
+    set_zero(m);
+    m.v[0] = m.v[8] = m.v[16] = R(1);
+}
+
+// Products.  Suffix n = operand as is, d = operand daggered.
+template <typename R>
+LGF_DEV M3<R> mul_nn(const M3<R>& a, const M3<R>& b) {
+    M3<R> o;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R ar = a.v[2 * (3 * r + k)], ai = a.v[2 * (3 * r + k) + 1];
+                R br = b.v[2 * (3 * k + c)], bi = b.v[2 * (3 * k + c) + 1];
+                x += ar * br - ai * bi;
+                y += ar * bi + ai * br;
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+    return o;
+}
+
+// a * b^dag : (a b^dag)_{rc} = sum_k a_{rk} conj(b_{ck})
+template <typename R>
+LGF_DEV M3<R> mul_nd(const M3<R>& a, const M3<R>& b) {
+    M3<R> o;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R ar = a.v[2 * (3 * r + k)], ai = a.v[2 * (3 * r + k) + 1];
+                R br = b.v[2 * (3 * c + k)], bi = -b.v[2 * (3 * c + k) + 1];
+                x += ar * br - ai * bi;
+                y += ar * bi + ai * br;
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+    return o;
+}
+
+// a^dag * b : sum_k conj(a_{kr}) b_{kc}
+template <typename R>
+LGF_DEV M3<R> mul_dn(const M3<R>& a, const M3<R>& b) {
+    M3<R> o;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                R ar = a.v[2 * (3 * k + r)], ai = -a.v[2 * (3 * k + r) + 1];
+                R br = b.v[2 * (3 * k + c)], bi = b.v[2 * (3 * k + c) + 1];
+                x += ar * br - ai * bi;
+                y += ar * bi + ai * br;
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+    return o;
+}
+
+template <typename R>
+LGF_DEV void add_to(M3<R>& acc, const M3<R>& b) {
+#pragma unroll
+    for (int i = 0; i < 18; ++i) acc.v[i] += b.v[i];
+}
+
+template <typename R>
+LGF_DEV R retr(const M3<R>& a) { return a.v[0] + a.v[8] + a.v[16]; }
+
+// Re Tr(a b) without forming the product
+template <typename R>
+LGF_DEV R retr_mul(const M3<R>& a, const M3<R>& b) {
+    R s = R(0);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            s += a.v[2 * (3 * r + k)] * b.v[2 * (3 * k + r)] - a.v[2 * (3 * r + k) + 1] * b.v[2 * (3 * k + r) + 1];
+    return s;
+}
+
+// Traceless anti-Hermitian part scaled: out = s * [ (W - W^dag) - 1/3 Tr(W - W^dag) ]
+template <typename R>
+LGF_DEV M3<R> ta_scaled(const M3<R>& W, R s) {
+    M3<R> o;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            o.v[2 * (3 * r + c)] = W.v[2 * (3 * r + c)] - W.v[2 * (3 * c + r)];
+            o.v[2 * (3 * r + c) + 1] = W.v[2 * (3 * r + c) + 1] + W.v[2 * (3 * c + r) + 1];
+        }
+    // the real part of the diagonal of W - W^dag is exactly zero
+    R tr_re = (o.v[0] + o.v[8] + o.v[16]) * (R(1) / R(3));
+    R tr_im = (o.v[1] + o.v[9] + o.v[17]) * (R(1) / R(3));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        o.v[8 * d] -= tr_re;
+        o.v[8 * d + 1] -= tr_im;
+    }
+#pragma unroll
+    for (int i = 0; i < 18; ++i) o.v[i] *= s;
+    return o;
+}
+
+}  // namespace lgf

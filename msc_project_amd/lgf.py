@@ -1,0 +1,269 @@
+"""ctypes binding of liblgf_flow.so (the C-ABI declared in include/lgf_flow.h).
+
+The library is the only compute path: importing this module does not touch
+the GPU, but every object that computes needs liblgf_flow.so and raises
+LgfError when it is missing, so nothing can silently fall back to the CPU.
+Link arrays are numpy float64 in the reference's saved-configuration order,
+shape (V, 4, 18) (see include/lgf_flow.h).
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liblgf_flow.so")
+
+FP64 = 0
+FP32 = 1
+MEASURE_DEFAULT = 0
+MEASURE_FP64 = 1
+
+STATUS = {0: "ok", 1: "bad argument", 2: "bad shape", 3: "I/O error", 4: "out of memory", 5: "HIP error",
+          6: "non-finite result", 7: "invalid state", 8: "admission refused", 9: "unsupported"}
+
+
+class LgfError(RuntimeError):
+    def __init__(self, code, msg=""):
+        self.code = code
+        super().__init__("lgf status %d (%s)%s" % (code, STATUS.get(code, "?"), (": " + msg) if msg else ""))
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+_ip = ctypes.POINTER(ctypes.c_int)
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("shape", ctypes.c_int * 4), ("precision", ctypes.c_int), ("beta", ctypes.c_double),
+                ("eps", ctypes.c_double), ("nsteps", ctypes.c_int), ("measure_every", ctypes.c_int),
+                ("links", _dp), ("measure_flags", ctypes.c_int)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int), ("nmeas", ctypes.c_int),
+                ("submit_s", ctypes.c_double), ("start_s", ctypes.c_double), ("finish_s", ctypes.c_double),
+                ("t", ctypes.c_double * 1024), ("E", ctypes.c_double * 1024), ("Q", ctypes.c_double * 1024)]
+
+
+# name -> (argtypes, restype); every symbol declared in include/lgf_flow.h
+SIGNATURES = {
+    "lgf_ctx_create": ([ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "lgf_ctx_destroy": ([_vp], ctypes.c_int),
+    "lgf_ctx_last_error": ([_vp], ctypes.c_char_p),
+    "lgf_ctx_stream": ([_vp, ctypes.POINTER(_vp)], ctypes.c_int),
+    "lgf_ctx_synchronize": ([_vp], ctypes.c_int),
+    "lgf_ctx_set_stage_variant": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "lgf_ctx_get_stage_variant": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
+    "lgf_num_stage_variants": ([ctypes.c_int, _ip], ctypes.c_int),
+    "lgf_field_footprint": ([_ip, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
+    "lgf_field_create": ([_vp, _ip, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "lgf_field_destroy": ([_vp], ctypes.c_int),
+    "lgf_field_shape": ([_vp, _ip, _ip], ctypes.c_int),
+    "lgf_field_upload": ([_vp, _dp], ctypes.c_int),
+    "lgf_field_download": ([_vp, _dp], ctypes.c_int),
+    "lgf_field_set_cold": ([_vp], ctypes.c_int),
+    "lgf_field_copy": ([_vp, _vp], ctypes.c_int),
+    "lgf_field_load_file": ([_vp, ctypes.c_char_p], ctypes.c_int),
+    "lgf_field_save_file": ([_vp, ctypes.c_char_p], ctypes.c_int),
+    "lgf_flow_steps": ([_vp, ctypes.c_double, ctypes.c_int], ctypes.c_int),
+    "lgf_measure": ([_vp, ctypes.c_int, _dp, _dp, _dp, _dp], ctypes.c_int),
+    "lgf_diagnostics": ([_vp, _dp, _dp, _dp], ctypes.c_int),
+    "lgf_batch_create": ([_vp, ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "lgf_batch_destroy": ([_vp], ctypes.c_int),
+    "lgf_batch_flow_steps": ([_vp, _dp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
+    "lgf_batch_measure": ([_vp, ctypes.c_int, _dp, _dp, _dp], ctypes.c_int),
+    "lgf_server_create": ([ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
+    "lgf_server_destroy": ([_vp], ctypes.c_int),
+    "lgf_server_submit": ([_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+    "lgf_server_wait": ([_vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(Result)], ctypes.c_int),
+    "lgf_server_now": ([_vp, _dp], ctypes.c_int),
+    "lgf_server_stats": ([_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _dp, _dp],
+                         ctypes.c_int),
+    "lgf_server_last_error": ([_vp], ctypes.c_char_p),
+}
+
+
+def lib():
+    """Load liblgf_flow.so; raises LgfError if it is not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LgfError(9, "liblgf_flow.so is not built (run __graft_entry__.build())")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_dp)
+
+
+def _shape(shape):
+    return (ctypes.c_int * 4)(*[int(s) for s in shape])
+
+
+def volume(shape):
+    v = 1
+    for s in shape:
+        v *= int(s)
+    return v
+
+
+class Context:
+    def __init__(self, device=0):
+        h = _vp()
+        rc = lib().lgf_ctx_create(device, ctypes.byref(h))
+        if rc:
+            raise LgfError(rc, "lgf_ctx_create(%d)" % device)
+        self.h = h
+
+    def check(self, rc, what=""):
+        if rc:
+            msg = lib().lgf_ctx_last_error(self.h)
+            raise LgfError(rc, (what + ": " if what else "") + (msg.decode() if msg else ""))
+
+    def stream(self):
+        s = _vp()
+        self.check(lib().lgf_ctx_stream(self.h, ctypes.byref(s)))
+        return s.value
+
+    def synchronize(self):
+        self.check(lib().lgf_ctx_synchronize(self.h))
+
+    def set_stage_variant(self, precision, variant):
+        self.check(lib().lgf_ctx_set_stage_variant(self.h, precision, variant))
+
+    def stage_variant(self, precision):
+        v = ctypes.c_int()
+        self.check(lib().lgf_ctx_get_stage_variant(self.h, precision, ctypes.byref(v)))
+        return v.value
+
+    def close(self):
+        if self.h:
+            lib().lgf_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def num_stage_variants(precision):
+    n = ctypes.c_int()
+    rc = lib().lgf_num_stage_variants(precision, ctypes.byref(n))
+    if rc:
+        raise LgfError(rc)
+    return n.value
+
+
+def footprint(shape, precision):
+    b = ctypes.c_size_t()
+    rc = lib().lgf_field_footprint(_shape(shape), precision, ctypes.byref(b))
+    if rc:
+        raise LgfError(rc, "footprint")
+    return b.value
+
+
+class Field:
+    """One lattice resident on the GPU."""
+
+    def __init__(self, ctx, shape, precision=FP64):
+        self.ctx = ctx
+        self.shape = tuple(int(s) for s in shape)
+        self.precision = precision
+        self.vol = volume(shape)
+        h = _vp()
+        ctx.check(lib().lgf_field_create(ctx.h, _shape(shape), precision, ctypes.byref(h)), "lgf_field_create")
+        self.h = h
+
+    def upload(self, links):
+        links = np.ascontiguousarray(links, dtype=np.float64)
+        assert links.size == self.vol * 72
+        self.ctx.check(lib().lgf_field_upload(self.h, _p(links)), "upload")
+
+    def download(self):
+        out = np.empty((self.vol, 4, 18), dtype=np.float64)
+        self.ctx.check(lib().lgf_field_download(self.h, _p(out)), "download")
+        return out
+
+    def set_cold(self):
+        self.ctx.check(lib().lgf_field_set_cold(self.h), "set_cold")
+
+    def copy_from(self, other):
+        self.ctx.check(lib().lgf_field_copy(self.h, other.h), "copy")
+
+    def load_file(self, path):
+        self.ctx.check(lib().lgf_field_load_file(self.h, path.encode()), "load_file")
+
+    def save_file(self, path):
+        self.ctx.check(lib().lgf_field_save_file(self.h, path.encode()), "save_file")
+
+    def flow(self, eps, nsteps):
+        self.ctx.check(lib().lgf_flow_steps(self.h, float(eps), int(nsteps)), "flow")
+
+    def measure(self, flags=MEASURE_DEFAULT, want_q=False):
+        """Returns (E, Q, Qabs, qraw or None)."""
+        E, Q, Qa = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        q = np.empty(self.vol) if want_q else None
+        self.ctx.check(lib().lgf_measure(self.h, flags, ctypes.byref(E), ctypes.byref(Q), ctypes.byref(Qa),
+                                         _p(q) if want_q else None), "measure")
+        return E.value, Q.value, Qa.value, q
+
+    def diagnostics(self):
+        p, mx, mn = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        self.ctx.check(lib().lgf_diagnostics(self.h, ctypes.byref(p), ctypes.byref(mx), ctypes.byref(mn)),
+                       "diagnostics")
+        return p.value, mx.value, mn.value
+
+    def close(self):
+        if self.h:
+            lib().lgf_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Batch:
+    """Several lattices (any shapes, one precision) flowed by shared launches."""
+
+    def __init__(self, ctx, fields):
+        self.ctx = ctx
+        self.fields = list(fields)
+        arr = (_vp * len(self.fields))(*[f.h for f in self.fields])
+        h = _vp()
+        ctx.check(lib().lgf_batch_create(ctx.h, arr, len(self.fields), ctypes.byref(h)), "lgf_batch_create")
+        self.h = h
+
+    def flow(self, eps, nsteps, use_graph=False):
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(eps, dtype=np.float64), (len(self.fields),)))
+        self.ctx.check(lib().lgf_batch_flow_steps(self.h, _p(e), int(nsteps), 1 if use_graph else 0), "batch flow")
+
+    def measure(self, flags=MEASURE_DEFAULT):
+        n = len(self.fields)
+        E, Q, Qa = np.empty(n), np.empty(n), np.empty(n)
+        self.ctx.check(lib().lgf_batch_measure(self.h, flags, _p(E), _p(Q), _p(Qa)), "batch measure")
+        return E, Q, Qa
+
+    def close(self):
+        if self.h:
+            lib().lgf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,207 @@
+"""GPU parity of the HIP path (liblgf_flow.so through its C-ABI) against the
+reference's golden vectors and the CPU oracle.  Bar: FP64 results within
+1e-12 relative in E and in Q (Q normalised by max(|Q|, sum_x|q(x)|/(4 pi^2)),
+since Q itself can be ~0); integer/layout work bit-exact.
+"""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+from msc_project_amd import lgf
+from oracle import oracle as O
+from tests.common import TOL_FP64, config1_inputs, golden, q_err, rel_err, small_input
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = lgf.Context(0)
+    yield c
+    c.close()
+
+
+def flow_measure(field, eps, steps, every, flags=lgf.MEASURE_DEFAULT):
+    t, E, Q, Qa = [0.0], [], [], []
+    e, q, qa, _ = field.measure(flags)
+    E.append(e), Q.append(q), Qa.append(qa)
+    tt = 0.0
+    for s in range(1, steps + 1):
+        field.flow(eps, 1)
+        tt += eps
+        if s % every == 0:
+            e, q, qa, _ = field.measure(flags)
+            t.append(tt), E.append(e), Q.append(q), Qa.append(qa)
+    return np.array(t), np.array(E), np.array(Q), np.array(Qa)
+
+
+def cold_links(shape):
+    U = O.empty_field(shape)
+    U[:, :, 0] = U[:, :, 8] = U[:, :, 16] = 1.0
+    return U
+
+
+def test_upload_download_roundtrip(ctx):
+    shape = (4, 6, 4, 8)
+    U = O.generate(shape, 6.0, seed=1, start="hot")
+    f = lgf.Field(ctx, shape, lgf.FP64)
+    f.upload(U)
+    assert np.array_equal(f.download(), U)
+    g = lgf.Field(ctx, shape, lgf.FP32)
+    g.upload(U)
+    assert np.array_equal(g.download(), U.astype(np.float32).astype(np.float64))
+
+
+def test_cold_start_is_fixed(ctx):
+    shape = (4, 4, 4, 6)
+    for prec in (lgf.FP64, lgf.FP32):
+        f = lgf.Field(ctx, shape, prec)
+        E, Q, Qa, _ = f.measure()
+        assert E == 0.0 and Q == 0.0
+        f.flow(0.02, 3)
+        assert np.array_equal(f.download(), cold_links(shape))
+
+
+@pytest.mark.parametrize("shape", [(4, 4, 4, 4), (6, 4, 8, 4), (4, 8, 4, 6)])
+def test_measure_matches_oracle(ctx, shape):
+    U = O.generate(shape, 6.0, seed=sum(shape), start="hot", sweeps=3)
+    f = lgf.Field(ctx, shape, lgf.FP64)
+    f.upload(U)
+    E, Q, Qa, qraw = f.measure(want_q=True)
+    Eo = O.energy_density(shape, U)
+    Qo, Qao, qxo = O.top_charge(shape, U, want_qx=True)
+    assert rel_err(E, Eo) < TOL_FP64
+    assert q_err(Q, Qo, Qao) < TOL_FP64
+    assert rel_err(Qa, Qao) < 1e-12
+    # per-site density: the oracle returns the reference's file scale pi^2/4
+    qo_raw = qxo / (np.pi ** 2 / 4)
+    assert np.max(np.abs(qraw - qo_raw)) < 1e-12 * np.max(np.abs(qo_raw))
+
+
+def test_flow_matches_oracle_links(ctx):
+    shape = (4, 6, 4, 4)
+    U = O.generate(shape, 6.0, seed=77, start="hot", sweeps=2)
+    f = lgf.Field(ctx, shape, lgf.FP64)
+    f.upload(U)
+    f.flow(0.02, 7)
+    for _ in range(7):
+        O.rk3_step(shape, U, 0.02)
+    assert np.max(np.abs(f.download() - U)) < 1e-13
+
+
+def test_small_golden(ctx):
+    d, U = small_input()
+    shape = tuple(d["shape"])
+    c = d["configs"][0]
+    f = lgf.Field(ctx, shape, lgf.FP64)
+    f.upload(U)
+    t, E, Q, Qa = flow_measure(f, d["eps"], d["steps"], d["every"])
+    assert np.allclose(t, c["t"], rtol=0, atol=1e-14)
+    assert rel_err(E, c["E"]) < TOL_FP64
+    assert q_err(Q, c["Q"], Qa) < TOL_FP64
+    fin = f.download().reshape(-1)
+    assert np.max(np.abs(fin[:18] - np.asarray(c["flowed_first_link"]))) < 1e-14
+    assert np.max(np.abs(fin[-18:] - np.asarray(c["flowed_last_link"]))) < 1e-14
+
+
+@pytest.fixture(scope="module")
+def config1():
+    return config1_inputs()
+
+
+def test_config1_golden_fp64(ctx, config1):
+    d, shape, cfgs = config1
+    for i, (U, c) in enumerate(zip(cfgs, d["configs"])):
+        assert hashlib.sha256(U.tobytes()).hexdigest() == d["input_sha256"][i]
+        f = lgf.Field(ctx, shape, lgf.FP64)
+        f.upload(U)
+        t, E, Q, Qa = flow_measure(f, d["eps"], d["steps"], d["every"])
+        print("config %d: max rel dE %.2e, max dQ/scale %.2e" % (i, rel_err(E, c["E"]), q_err(Q, c["Q"], Qa)))
+        assert rel_err(E, c["E"]) < TOL_FP64
+        assert q_err(Q, c["Q"], Qa) < TOL_FP64
+
+
+def test_batch_same_alone_and_in_mix(ctx):
+    """each lattice's results are bitwise the same flowed alone or batched with others"""
+    shapes = [(4, 4, 4, 4), (6, 4, 4, 8), (4, 4, 4, 4), (8, 6, 4, 4)]
+    eps = [0.02, 0.01, 0.03, 0.02]
+    inputs = [O.generate(s, 6.0, seed=100 + i, start="hot", sweeps=2) for i, s in enumerate(shapes)]
+    for prec in (lgf.FP64, lgf.FP32):
+        alone = []
+        for U, s, e in zip(inputs, shapes, eps):
+            f = lgf.Field(ctx, s, prec)
+            f.upload(U)
+            f.flow(e, 5)
+            alone.append((f.download(), f.measure()[:3]))
+        fields = [lgf.Field(ctx, s, prec) for s in shapes]
+        for f, U in zip(fields, inputs):
+            f.upload(U)
+        b = lgf.Batch(ctx, fields)
+        b.flow(eps, 5, use_graph=False)
+        E, Q, Qa = b.measure()
+        for i, f in enumerate(fields):
+            assert np.array_equal(f.download(), alone[i][0])
+            assert (E[i], Q[i], Qa[i]) == alone[i][1]
+
+
+def test_graph_replay_matches_direct(ctx):
+    shapes = [(4, 4, 4, 4), (6, 4, 4, 8)]
+    inputs = [O.generate(s, 6.0, seed=200 + i, start="hot", sweeps=2) for i, s in enumerate(shapes)]
+    res = []
+    for use_graph in (False, True):
+        fields = [lgf.Field(ctx, s, lgf.FP64) for s in shapes]
+        for f, U in zip(fields, inputs):
+            f.upload(U)
+        b = lgf.Batch(ctx, fields)
+        out = []
+        for rep in range(3):  # several replays, eager work between them
+            b.flow([0.02, 0.01], 4, use_graph=use_graph)
+            out.append(b.measure())
+        b.flow([0.02, 0.01], 3, use_graph=use_graph)  # odd count: direct step after a replay
+        out.append(b.measure())
+        res.append((out, [f.download() for f in fields]))
+    for a, b_ in zip(res[0][0], res[1][0]):
+        for x, y in zip(a, b_):
+            assert np.array_equal(x, y)
+    for x, y in zip(res[0][1], res[1][1]):
+        assert np.array_equal(x, y)
+
+
+def test_file_roundtrip(ctx, tmp_path):
+    d, U = small_input()
+    shape = tuple(d["shape"])
+    from tests.common import GOLDEN
+    f = lgf.Field(ctx, shape, lgf.FP64)
+    f.load_file(os.path.join(GOLDEN, d["input_file"]))
+    assert np.array_equal(f.download(), U)
+    p = str(tmp_path / "cfg.bin")
+    f.save_file(p)
+    with open(p, "rb") as fh, open(os.path.join(GOLDEN, d["input_file"]), "rb") as gh:
+        assert fh.read() == gh.read()
+    with pytest.raises(lgf.LgfError) as e:
+        f.load_file(str(tmp_path / "missing.bin"))
+    assert e.value.code == 3
+
+
+def test_bad_shape_rejected(ctx):
+    with pytest.raises(lgf.LgfError) as e:
+        lgf.Field(ctx, (4, 5, 4, 4), lgf.FP64)
+    assert e.value.code == 2
+
+
+def test_fp32_tracks_fp64(ctx, config1):
+    """mixed precision vs the FP64 path on the config-1 inputs"""
+    d, shape, cfgs = config1
+    U = cfgs[0]
+    f64 = lgf.Field(ctx, shape, lgf.FP64)
+    f32 = lgf.Field(ctx, shape, lgf.FP32)
+    f64.upload(U)
+    f32.upload(U)
+    t, E64, Q64, Qa = flow_measure(f64, d["eps"], d["steps"], d["every"])
+    _, E32, Q32, _ = flow_measure(f32, d["eps"], d["steps"], d["every"], flags=lgf.MEASURE_FP64)
+    dE = np.abs(t ** 2 * (E32 - E64)) / np.abs(t ** 2 * E64 + 1e-300)
+    print("fp32: max rel d(t^2E) %.2e  max |dQ| %.2e" % (dE[1:].max(), np.abs(Q32 - Q64).max()))
+    assert np.abs(Q32 - Q64).max() < 1e-4
+    assert dE[1:].max() < 1e-5

@@ -1,0 +1,48 @@
+"""CPU checks of the C-ABI library: it loads without a GPU and exports every
+function include/lgf_flow.h declares; host-side validation needs no GPU."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from msc_project_amd import lgf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "lgf_flow.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(lgf_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_header_declares_api():
+    names = declared_functions()
+    assert len(names) >= 30
+    assert set(names) == set(lgf.SIGNATURES), set(names) ^ set(lgf.SIGNATURES)
+
+
+def test_library_exports_every_declared_symbol():
+    L = ctypes.CDLL(lgf.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(L, name), name
+
+
+def test_footprint_and_shape_validation_without_gpu():
+    assert lgf.footprint((4, 4, 4, 4), lgf.FP64) == 3 * 72 * 256 * 8
+    assert lgf.footprint((24, 24, 24, 24), lgf.FP32) == 3 * 72 * 331776 * 4
+    with pytest.raises(lgf.LgfError) as e:
+        lgf.footprint((4, 3, 4, 4), lgf.FP64)
+    assert e.value.code == 2
+    n = lgf.num_stage_variants(lgf.FP64)
+    assert n >= 1 and lgf.num_stage_variants(lgf.FP32) >= 1
+
+
+def test_context_without_gpu_fails_loudly():
+    import subprocess
+    import sys
+    # a fresh process: no GPU in this container means lgf_ctx_create must error, never fall back
+    code = "from msc_project_amd import lgf\ntry:\n    lgf.Context(0)\n    print('GPU')\nexcept lgf.LgfError as e:\n    print('ERR', e.code)\n"
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
+    assert out.stdout.strip() in ("GPU", "ERR 5")
