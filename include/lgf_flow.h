@@ -30,6 +30,9 @@
  *       no counterpart in the reference (a batch CLI tool); a continuous-
  *       batching request loop for flow jobs.
  *
+ * Lifetimes: destroy batches before the fields they hold and fields before
+ * their context; out-of-order destroys return LGF_E_STATE and change nothing.
+ *
  * Threading: calls on one context are not reentrant; use one context per
  * host thread.  The server runs its own scheduler thread and is safe to
  * submit to from any thread.
@@ -94,6 +97,9 @@ int lgf_field_shape(const lgf_field* f, int shape[4], int* precision);
 int lgf_field_upload(lgf_field* f, const double* links);
 int lgf_field_download(lgf_field* f, double* links);
 int lgf_field_set_cold(lgf_field* f);
+/* seeded Haar-random SU(3) links (a "hot start"; same Philox4x32-10 counters
+ * as the CPU oracle's, values equal to rounding) */
+int lgf_field_set_random(lgf_field* f, uint64_t seed);
 /* copy links of src into dst (same shape; precision converted) */
 int lgf_field_copy(lgf_field* dst, const lgf_field* src);
 /* read/write a configuration file of the reference's format at `path`
@@ -127,7 +133,9 @@ typedef struct {
     double eps;
     int nsteps;
     int measure_every;      /* measure at t=0 and every k steps; 0 = only at t=0 and the end */
-    const double* links;    /* initial links in file order (copied at submit) or NULL for cold start */
+    const double* links;    /* initial links in file order (copied at submit), or NULL: */
+    int start;              /*   links == NULL: 0 = cold start, 1 = seeded Haar-random links */
+    uint64_t seed;          /*   seed of the Haar-random start */
     int measure_flags;      /* lgf_measure_flags */
 } lgf_request;
 

@@ -86,6 +86,8 @@ extern "C" int lgf_ctx_create(int device, lgf_ctx** out) {
 
 extern "C" int lgf_ctx_destroy(lgf_ctx* ctx) {
     if (!ctx) return LGF_E_ARG;
+    if (ctx->live_fields || ctx->live_batches)
+        return lgf_fail(ctx, LGF_E_STATE, "destroy the context's fields and batches first");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->staging) (void)hipFree(ctx->staging);
@@ -179,6 +181,7 @@ extern "C" int lgf_field_create(lgf_ctx* ctx, const int shape[4], int precision,
     f->links[1] = p[1];
     f->acc = p[2];
     f->cur = 0;
+    ctx->live_fields++;
     rc = lgf_field_set_cold(f);
     if (rc) {
         lgf_field_destroy(f);
@@ -191,9 +194,11 @@ extern "C" int lgf_field_create(lgf_ctx* ctx, const int shape[4], int precision,
 extern "C" int lgf_field_destroy(lgf_field* f) {
     if (!f) return LGF_E_ARG;
     lgf_ctx* ctx = f->ctx;
+    if (f->in_batches) return lgf_fail(ctx, LGF_E_STATE, "destroy the batches holding this field first");
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
     if (f->solo) lgf_batch_destroy(f->solo);
+    ctx->live_fields--;
     (void)hipFree(f->links[0]);
     (void)hipFree(f->links[1]);
     (void)hipFree(f->acc);
@@ -215,6 +220,13 @@ extern "C" int lgf_field_set_cold(lgf_field* f) {
     lgf_ctx* ctx = f->ctx;
     HIPTRY(ctx, launch_cold(f->prec, f->links[f->cur], f->stride, f->vol, ctx->stream));
     HIPTRY(ctx, launch_cold(f->prec, f->acc, f->stride, f->vol, ctx->stream));
+    return LGF_OK;
+}
+
+extern "C" int lgf_field_set_random(lgf_field* f, uint64_t seed) {
+    if (!f) return LGF_E_ARG;
+    lgf_ctx* ctx = f->ctx;
+    HIPTRY(ctx, launch_hot(f->prec, f->links[f->cur], f->stride, f->vol, seed, ctx->stream));
     return LGF_OK;
 }
 
@@ -325,6 +337,9 @@ static int solo(lgf_field* f, lgf_batch** b) {
     if (!f->solo) {
         int rc = lgf_batch_create(f->ctx, &f, 1, &f->solo);
         if (rc) return rc;
+        f->solo->is_solo = true;
+        f->in_batches--;
+        f->ctx->live_batches--;
     }
     *b = f->solo;
     return LGF_OK;
@@ -418,6 +433,8 @@ extern "C" int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfie
         e = hipMemcpy(b->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(b->d_desc, b->h_desc.data(), sizeof(FieldDesc) * nfields, hipMemcpyHostToDevice);
+    for (lgf_field* f : b->fields) f->in_batches++;
+    ctx->live_batches++;
     if (e != hipSuccess) {
         lgf_batch_destroy(b);
         return lgf_hip_fail(ctx, e, "batch allocation");
@@ -428,6 +445,10 @@ extern "C" int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfie
 
 extern "C" int lgf_batch_destroy(lgf_batch* b) {
     if (!b) return LGF_E_ARG;
+    if (!b->is_solo) {
+        for (lgf_field* f : b->fields) f->in_batches--;
+        b->ctx->live_batches--;
+    }
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
     for (int i = 0; i < 2; ++i)

@@ -17,6 +17,8 @@ struct lgf_ctx {
     int stage_variant[2] = {0, 0};
     void* staging = nullptr;  // device staging chunk for uploads/downloads
     void* pinned = nullptr;   // pinned host staging chunk
+    int live_fields = 0;      // fields and user batches not yet destroyed
+    int live_batches = 0;
 };
 
 struct lgf_field {
@@ -31,6 +33,7 @@ struct lgf_field {
     double* qx = nullptr;  // lazily allocated per-site charge density
     int cur = 0;           // links[cur] holds the current configuration
     lgf_batch* solo = nullptr;
+    int in_batches = 0;    // user batches that reference this field
 };
 
 struct lgf_batch {
@@ -47,6 +50,7 @@ struct lgf_batch {
     int ping = 0;
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // two RK3 steps, by starting ping
     int graph_variant[2] = {-1, -1};
+    bool is_solo = false;  // a field's private one-field batch
 };
 
 int lgf_fail(lgf_ctx* ctx, int code, const std::string& msg);

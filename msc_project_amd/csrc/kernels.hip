@@ -24,6 +24,7 @@
 #include "expm.hpp"
 #include "kernels.hpp"
 #include "lattice.hpp"
+#include "philox.hpp"
 #include "su3.hpp"
 
 namespace lgf {
@@ -426,6 +427,27 @@ __global__ __launch_bounds__(256) void k_convert(const SI* __restrict__ src, SO*
     if (i < n) dst[i] = (SO)src[i];
 }
 
+// Seeded Haar-random SU(3) links: rows 0 and 1 from complex Gaussians
+// (Box-Muller on Philox uniforms, counter (link, i, ~0, 3) as in the CPU
+// oracle's hot start), orthonormalised, row 2 = conj(row0 x row1).
+template <typename S>
+__global__ __launch_bounds__(256) void k_hot(S* __restrict__ dst, int64_t stride, int vol, uint64_t seed) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // link = 4*site + mu
+    if (l >= (int64_t)vol * 4) return;
+    const int s = (int)(l >> 2), mu = (int)(l & 3);
+    M3<double> m;
+    set_zero(m);
+    for (int i = 0; i < 6; ++i) {
+        double u[2];
+        uniform2(seed, (uint32_t)l, (uint32_t)i, 0xFFFFFFFFu, 3u, u);
+        const double r = sqrt(-2.0 * log(u[0]));
+        m.v[2 * i] = r * cos(2.0 * M_PI * u[1]);
+        m.v[2 * i + 1] = r * sin(2.0 * M_PI * u[1]);
+    }
+    reunitarize(m);
+    store_link<S, double>(dst, stride, mu, s, m);
+}
+
 template <typename S>
 __global__ __launch_bounds__(256) void k_cold(S* __restrict__ dst, int64_t stride, int vol) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -538,6 +560,15 @@ hipError_t launch_convert(int prec_in, const void* src, int prec_out, void* dst,
         hipLaunchKernelGGL((k_convert<float, double>), grid, block, 0, stream, (const float*)src, (double*)dst, n);
     else
         hipLaunchKernelGGL((k_convert<float, float>), grid, block, 0, stream, (const float*)src, (float*)dst, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_hot(int prec, void* dst, int64_t stride, int vol, uint64_t seed, hipStream_t stream) {
+    dim3 grid((unsigned)(((int64_t)vol * 4 + 255) / 256)), block(256);
+    if (prec == kPrecFP64)
+        hipLaunchKernelGGL(k_hot<double>, grid, block, 0, stream, (double*)dst, stride, vol, seed);
+    else
+        hipLaunchKernelGGL(k_hot<float>, grid, block, 0, stream, (float*)dst, stride, vol, seed);
     return hipGetLastError();
 }
 

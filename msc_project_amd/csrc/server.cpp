@@ -1,13 +1,369 @@
-// Continuous-batching flow server (see include/lgf_flow.h).
+// Continuous-batching flow server (include/lgf_flow.h, lgf_server_*).
+//
+// No counterpart in the reference, which flows saved configurations one after
+// another from a CLI (LGF/LGF.cpp:163-196).  Here flow jobs arrive at any
+// time; one scheduler thread owns the GPU context and loops:
+//   1. admit queued jobs, first come first served, while their device
+//      footprint (2 link buffers + RK3 accumulator) fits the budget;
+//   2. run k RK3 steps for every active job with one set of launches per
+//      precision (a batch over all active lattices, whatever their shapes;
+//      k = steps to the nearest measurement or completion, at most kMaxGroup,
+//      so new arrivals join within a few steps);
+//   3. measure the jobs that are due, retire finished jobs and free their
+//      memory, which lets the next queued jobs in.
+// Every job's arithmetic is independent of the other jobs in the batch, and
+// its reductions run over its own tiles in a fixed order, so a job's results
+// are bitwise the same alone and in any mix.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
 #include "../../include/lgf_flow.h"
 
-extern "C" int lgf_server_create(int, size_t, int, lgf_server** out) {
-    if (out) *out = nullptr;
-    return LGF_E_UNSUPPORTED;
+namespace {
+
+constexpr int kMaxGroup = 8;      // steps between admission points
+constexpr int kMaxMeasurements = 1024;
+
+struct Job {
+    uint64_t id = 0;
+    lgf_request req{};
+    std::vector<double> links;  // copy of the caller's initial links (may be empty)
+    size_t bytes = 0;
+    lgf_field* field = nullptr;
+    int done = 0;
+    double t = 0.0;
+    int status = LGF_OK;
+    double submit_s = 0, start_s = 0, finish_s = 0;
+    std::vector<double> tm, E, Q;
+};
+
+using JobP = std::shared_ptr<Job>;
+
+}  // namespace
+
+struct lgf_server {
+    int device = 0;
+    size_t budget = 0;
+    int max_active = 0;
+    lgf_ctx* ctx = nullptr;
+    std::thread thread;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::deque<JobP> pending;
+    std::map<uint64_t, JobP> finished;
+    uint64_t next_id = 1;
+    bool stop = false;
+    std::string err;
+    std::chrono::steady_clock::time_point t0;
+    // counters (guarded by mu)
+    uint64_t jobs_done = 0, batch_steps = 0;
+    double link_updates = 0.0, busy_s = 0.0;
+
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void set_err(const std::string& m) {
+        std::lock_guard<std::mutex> lk(mu);
+        err = m;
+    }
+    void run();
+};
+
+static int64_t volume_of(const int* s) { return (int64_t)s[0] * s[1] * s[2] * s[3]; }
+
+static void measure_job(lgf_server* S, Job& j) {
+    double E = 0, Q = 0, Qa = 0;
+    int rc = lgf_measure(j.field, j.req.measure_flags, &E, &Q, &Qa, nullptr);
+    if (rc != LGF_OK && rc != LGF_E_NAN) {
+        j.status = rc;
+        S->set_err(lgf_ctx_last_error(S->ctx));
+        return;
+    }
+    if ((int)j.tm.size() < kMaxMeasurements) {
+        j.tm.push_back(j.t);
+        j.E.push_back(E);
+        j.Q.push_back(Q);
+    }
 }
-extern "C" int lgf_server_destroy(lgf_server*) { return LGF_E_UNSUPPORTED; }
-extern "C" int lgf_server_submit(lgf_server*, const lgf_request*, uint64_t*) { return LGF_E_UNSUPPORTED; }
-extern "C" int lgf_server_wait(lgf_server*, uint64_t, double, lgf_result*) { return LGF_E_UNSUPPORTED; }
-extern "C" int lgf_server_now(lgf_server*, double*) { return LGF_E_UNSUPPORTED; }
-extern "C" int lgf_server_stats(lgf_server*, uint64_t*, uint64_t*, double*, double*) { return LGF_E_UNSUPPORTED; }
-extern "C" const char* lgf_server_last_error(const lgf_server*) { return "not built"; }
+
+static bool due(const Job& j) {
+    const int every = j.req.measure_every;
+    return j.done == j.req.nsteps || (every > 0 && j.done % every == 0);
+}
+
+void lgf_server::run() {
+    std::vector<JobP> active;
+    lgf_batch* batch[2] = {nullptr, nullptr};
+    bool dirty = true;
+    while (true) {
+        std::vector<JobP> admitted;
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
+            if (stop) break;
+            size_t used = 0;
+            for (auto& j : active) used += j->bytes;
+            while (!pending.empty()) {
+                JobP j = pending.front();
+                if (used + j->bytes > budget) break;
+                if (max_active > 0 && (int)(active.size() + admitted.size()) >= max_active) break;
+                pending.pop_front();
+                used += j->bytes;
+                admitted.push_back(j);
+            }
+        }
+        for (auto& j : admitted) {
+            j->start_s = now();
+            int rc = lgf_field_create(ctx, j->req.shape, j->req.precision, &j->field);
+            if (rc == LGF_OK) {
+                if (!j->links.empty())
+                    rc = lgf_field_upload(j->field, j->links.data());
+                else if (j->req.start == 1)
+                    rc = lgf_field_set_random(j->field, j->req.seed);
+                j->links.clear();
+                j->links.shrink_to_fit();
+            }
+            if (rc == LGF_OK) {
+                measure_job(this, *j);  // t = 0 (LGF/LGF.cpp:178-184)
+                rc = j->status;
+            }
+            if (rc != LGF_OK || j->req.nsteps == 0) {
+                j->status = rc;
+                if (rc != LGF_OK) set_err(lgf_ctx_last_error(ctx));
+                if (j->field) lgf_field_destroy(j->field);
+                j->field = nullptr;
+                j->finish_s = now();
+                std::lock_guard<std::mutex> lk(mu);
+                finished[j->id] = j;
+                jobs_done++;
+                cv_done.notify_all();
+                continue;
+            }
+            active.push_back(j);
+            dirty = true;
+        }
+        if (active.empty()) continue;
+        const double tb = now();
+        if (dirty) {
+            for (int p = 0; p < 2; ++p) {
+                if (batch[p]) lgf_batch_destroy(batch[p]);
+                batch[p] = nullptr;
+                std::vector<lgf_field*> fs;
+                for (auto& j : active)
+                    if (j->req.precision == p) fs.push_back(j->field);
+                if (!fs.empty()) lgf_batch_create(ctx, fs.data(), (int)fs.size(), &batch[p]);
+            }
+            dirty = false;
+        }
+        // steps to the nearest event of any active job
+        int k = kMaxGroup;
+        for (auto& j : active) {
+            int to_end = j->req.nsteps - j->done;
+            int every = j->req.measure_every;
+            int to_meas = every > 0 ? every - j->done % every : to_end;
+            k = std::min(k, std::min(to_end, to_meas));
+        }
+        k = std::max(k, 1);
+        double lu = 0.0;
+        for (int p = 0; p < 2; ++p) {
+            if (!batch[p]) continue;
+            std::vector<double> eps;
+            for (auto& j : active)
+                if (j->req.precision == p) {
+                    eps.push_back(j->req.eps);
+                    lu += 4.0 * (double)volume_of(j->req.shape) * k;
+                }
+            int rc = lgf_batch_flow_steps(batch[p], eps.data(), k, 1);
+            if (rc != LGF_OK) {
+                set_err(lgf_ctx_last_error(ctx));
+                for (auto& j : active)
+                    if (j->req.precision == p) j->status = rc;
+            }
+        }
+        std::vector<JobP> still, done_now;
+        for (auto& j : active) {
+            for (int s = 0; s < k; ++s) j->t += j->req.eps;  // the reference's t += eps per step
+            j->done += k;
+            if (j->status == LGF_OK && due(*j)) measure_job(this, *j);
+            if (j->done >= j->req.nsteps || j->status != LGF_OK)
+                done_now.push_back(j);
+            else
+                still.push_back(j);
+        }
+        if (!done_now.empty()) {
+            // batches hold the fields: release them before the fields
+            for (int p = 0; p < 2; ++p) {
+                if (batch[p]) lgf_batch_destroy(batch[p]);
+                batch[p] = nullptr;
+            }
+            dirty = true;
+            for (auto& j : done_now) {
+                lgf_field_destroy(j->field);
+                j->field = nullptr;
+                j->finish_s = now();
+                std::lock_guard<std::mutex> lk(mu);
+                finished[j->id] = j;
+                jobs_done++;
+                cv_done.notify_all();
+            }
+        }
+        active.swap(still);
+        lgf_ctx_synchronize(ctx);
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            batch_steps += k;
+            link_updates += lu;
+            busy_s += now() - tb;
+        }
+    }
+    for (int p = 0; p < 2; ++p)
+        if (batch[p]) lgf_batch_destroy(batch[p]);
+    std::lock_guard<std::mutex> lk(mu);
+    for (auto& j : active) {
+        if (j->field) lgf_field_destroy(j->field);
+        j->field = nullptr;
+        j->status = LGF_E_STATE;
+        finished[j->id] = j;
+    }
+    for (auto& j : pending) {
+        j->status = LGF_E_STATE;
+        finished[j->id] = j;
+    }
+    pending.clear();
+    cv_done.notify_all();
+}
+
+extern "C" int lgf_server_create(int device, size_t budget_bytes, int max_active, lgf_server** out) {
+    if (!out || max_active < 0) return LGF_E_ARG;
+    *out = nullptr;
+    lgf_ctx* ctx = nullptr;
+    int rc = lgf_ctx_create(device, &ctx);
+    if (rc) return rc;
+    if (budget_bytes == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+            lgf_ctx_destroy(ctx);
+            return LGF_E_HIP;
+        }
+        budget_bytes = (size_t)(0.8 * (double)fr);
+    }
+    lgf_server* s = new lgf_server();
+    s->device = device;
+    s->budget = budget_bytes;
+    s->max_active = max_active;
+    s->ctx = ctx;
+    s->t0 = std::chrono::steady_clock::now();
+    s->thread = std::thread([s] {
+        (void)hipSetDevice(s->device);
+        s->run();
+    });
+    *out = s;
+    return LGF_OK;
+}
+
+extern "C" int lgf_server_destroy(lgf_server* s) {
+    if (!s) return LGF_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->thread.joinable()) s->thread.join();
+    lgf_ctx_destroy(s->ctx);
+    delete s;
+    return LGF_OK;
+}
+
+extern "C" int lgf_server_submit(lgf_server* s, const lgf_request* req, uint64_t* id) {
+    if (!s || !req || !id) return LGF_E_ARG;
+    size_t bytes = 0;
+    int rc = lgf_field_footprint(req->shape, req->precision, &bytes);
+    if (rc) return rc;
+    if (req->nsteps < 0 || req->measure_every < 0 || !(req->eps == req->eps)) return LGF_E_ARG;
+    if (bytes > s->budget) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = "request needs " + std::to_string(bytes) + " bytes of device memory, budget is " +
+                 std::to_string(s->budget);
+        return LGF_E_ADMISSION;
+    }
+    auto j = std::make_shared<Job>();
+    j->req = *req;
+    j->bytes = bytes;
+    if (req->links) j->links.assign(req->links, req->links + (size_t)volume_of(req->shape) * 72);
+    j->req.links = nullptr;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop) return LGF_E_STATE;
+        j->id = s->next_id++;
+        j->submit_s = s->now();
+        s->pending.push_back(j);
+        *id = j->id;
+    }
+    s->cv_work.notify_all();
+    return LGF_OK;
+}
+
+extern "C" int lgf_server_wait(lgf_server* s, uint64_t id, double timeout_s, lgf_result* out) {
+    if (!s || !out) return LGF_E_ARG;
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto ready = [&] { return s->finished.count(id) > 0; };
+    if (timeout_s < 0) {
+        s->cv_done.wait(lk, ready);
+    } else if (!s->cv_done.wait_for(lk, std::chrono::duration<double>(timeout_s), ready)) {
+        return LGF_E_STATE;
+    }
+    JobP j = s->finished[id];
+    s->finished.erase(id);
+    lk.unlock();
+    std::memset(out, 0, sizeof *out);
+    out->id = j->id;
+    out->status = j->status;
+    out->nmeas = (int)j->tm.size();
+    out->submit_s = j->submit_s;
+    out->start_s = j->start_s;
+    out->finish_s = j->finish_s;
+    for (int i = 0; i < out->nmeas; ++i) {
+        out->t[i] = j->tm[i];
+        out->E[i] = j->E[i];
+        out->Q[i] = j->Q[i];
+    }
+    return LGF_OK;
+}
+
+extern "C" int lgf_server_now(lgf_server* s, double* now_s) {
+    if (!s || !now_s) return LGF_E_ARG;
+    *now_s = s->now();
+    return LGF_OK;
+}
+
+extern "C" int lgf_server_stats(lgf_server* s, uint64_t* jobs_done, uint64_t* batch_steps, double* link_updates,
+                                double* busy_s) {
+    if (!s) return LGF_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (jobs_done) *jobs_done = s->jobs_done;
+    if (batch_steps) *batch_steps = s->batch_steps;
+    if (link_updates) *link_updates = s->link_updates;
+    if (busy_s) *busy_s = s->busy_s;
+    return LGF_OK;
+}
+
+// copied under the lock into a per-thread buffer: the scheduler may overwrite err
+extern "C" const char* lgf_server_last_error(const lgf_server* s) {
+    static thread_local std::string copy;
+    if (!s) return "null server";
+    lgf_server* m = const_cast<lgf_server*>(s);
+    std::lock_guard<std::mutex> lk(m->mu);
+    copy = m->err;
+    return copy.c_str();
+}

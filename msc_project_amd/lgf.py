@@ -6,8 +6,10 @@ LgfError when it is missing, so nothing can silently fall back to the CPU.
 Link arrays are numpy float64 in the reference's saved-configuration order,
 shape (V, 4, 18) (see include/lgf_flow.h).
 """
+import atexit
 import ctypes
 import os
+import weakref
 
 import numpy as np
 
@@ -38,7 +40,7 @@ _ip = ctypes.POINTER(ctypes.c_int)
 class Request(ctypes.Structure):
     _fields_ = [("shape", ctypes.c_int * 4), ("precision", ctypes.c_int), ("beta", ctypes.c_double),
                 ("eps", ctypes.c_double), ("nsteps", ctypes.c_int), ("measure_every", ctypes.c_int),
-                ("links", _dp), ("measure_flags", ctypes.c_int)]
+                ("links", _dp), ("start", ctypes.c_int), ("seed", ctypes.c_uint64), ("measure_flags", ctypes.c_int)]
 
 
 class Result(ctypes.Structure):
@@ -64,6 +66,7 @@ SIGNATURES = {
     "lgf_field_upload": ([_vp, _dp], ctypes.c_int),
     "lgf_field_download": ([_vp, _dp], ctypes.c_int),
     "lgf_field_set_cold": ([_vp], ctypes.c_int),
+    "lgf_field_set_random": ([_vp, ctypes.c_uint64], ctypes.c_int),
     "lgf_field_copy": ([_vp, _vp], ctypes.c_int),
     "lgf_field_load_file": ([_vp, ctypes.c_char_p], ctypes.c_int),
     "lgf_field_save_file": ([_vp, ctypes.c_char_p], ctypes.c_int),
@@ -116,6 +119,19 @@ def volume(shape):
     return v
 
 
+_live = weakref.WeakSet()  # contexts and servers still open
+
+
+@atexit.register
+def _close_all():
+    # release device objects in dependency order before the HIP runtime goes away
+    for obj in list(_live):
+        try:
+            obj.close()
+        except Exception:
+            pass
+
+
 class Context:
     def __init__(self, device=0):
         h = _vp()
@@ -123,6 +139,9 @@ class Context:
         if rc:
             raise LgfError(rc, "lgf_ctx_create(%d)" % device)
         self.h = h
+        self._fields = weakref.WeakSet()
+        self._batches = weakref.WeakSet()
+        _live.add(self)
 
     def check(self, rc, what=""):
         if rc:
@@ -147,6 +166,10 @@ class Context:
 
     def close(self):
         if self.h:
+            for b in list(self._batches):
+                b.close()
+            for f in list(self._fields):
+                f.close()
             lib().lgf_ctx_destroy(self.h)
             self.h = None
 
@@ -184,6 +207,8 @@ class Field:
         h = _vp()
         ctx.check(lib().lgf_field_create(ctx.h, _shape(shape), precision, ctypes.byref(h)), "lgf_field_create")
         self.h = h
+        self._batches = weakref.WeakSet()
+        ctx._fields.add(self)
 
     def upload(self, links):
         links = np.ascontiguousarray(links, dtype=np.float64)
@@ -197,6 +222,9 @@ class Field:
 
     def set_cold(self):
         self.ctx.check(lib().lgf_field_set_cold(self.h), "set_cold")
+
+    def set_random(self, seed):
+        self.ctx.check(lib().lgf_field_set_random(self.h, int(seed)), "set_random")
 
     def copy_from(self, other):
         self.ctx.check(lib().lgf_field_copy(self.h, other.h), "copy")
@@ -226,7 +254,10 @@ class Field:
 
     def close(self):
         if self.h:
-            lib().lgf_field_destroy(self.h)
+            for b in list(self._batches):
+                b.close()
+            if self.ctx.h:
+                self.ctx.check(lib().lgf_field_destroy(self.h), "lgf_field_destroy")
             self.h = None
 
     def __del__(self):
@@ -246,6 +277,9 @@ class Batch:
         h = _vp()
         ctx.check(lib().lgf_batch_create(ctx.h, arr, len(self.fields), ctypes.byref(h)), "lgf_batch_create")
         self.h = h
+        ctx._batches.add(self)
+        for f in self.fields:
+            f._batches.add(self)
 
     def flow(self, eps, nsteps, use_graph=False):
         e = np.ascontiguousarray(np.broadcast_to(np.asarray(eps, dtype=np.float64), (len(self.fields),)))
@@ -259,7 +293,74 @@ class Batch:
 
     def close(self):
         if self.h:
-            lib().lgf_batch_destroy(self.h)
+            if self.ctx.h:
+                lib().lgf_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous-batching flow server (lgf_server_*)."""
+
+    def __init__(self, device=0, budget_bytes=0, max_active=0):
+        h = _vp()
+        rc = lib().lgf_server_create(device, budget_bytes, max_active, ctypes.byref(h))
+        if rc:
+            raise LgfError(rc, "lgf_server_create")
+        self.h = h
+        _live.add(self)
+
+    def _check(self, rc, what):
+        if rc:
+            msg = lib().lgf_server_last_error(self.h)
+            raise LgfError(rc, what + ": " + (msg.decode() if msg else ""))
+
+    def submit(self, shape, eps, nsteps, measure_every, precision=FP32, links=None, seed=None, beta=6.0,
+               measure_flags=MEASURE_FP64):
+        r = Request()
+        r.shape[:] = [int(s) for s in shape]
+        r.precision, r.beta, r.eps, r.nsteps, r.measure_every = precision, beta, eps, nsteps, measure_every
+        r.measure_flags = measure_flags
+        if links is not None:
+            links = np.ascontiguousarray(links, dtype=np.float64)
+            r.links = _p(links)
+        else:
+            r.links = None
+            r.start = 1 if seed is not None else 0
+            r.seed = int(seed or 0)
+        i = ctypes.c_uint64()
+        self._check(lib().lgf_server_submit(self.h, ctypes.byref(r), ctypes.byref(i)), "submit")
+        return i.value
+
+    def wait(self, job_id, timeout=-1.0):
+        """Returns dict(status, t, E, Q, submit_s, start_s, finish_s)."""
+        res = Result()
+        self._check(lib().lgf_server_wait(self.h, job_id, float(timeout), ctypes.byref(res)), "wait")
+        n = res.nmeas
+        return {"id": res.id, "status": res.status, "t": np.array(res.t[:n]), "E": np.array(res.E[:n]),
+                "Q": np.array(res.Q[:n]), "submit_s": res.submit_s, "start_s": res.start_s,
+                "finish_s": res.finish_s}
+
+    def now(self):
+        t = ctypes.c_double()
+        self._check(lib().lgf_server_now(self.h, ctypes.byref(t)), "now")
+        return t.value
+
+    def stats(self):
+        jd, bs = ctypes.c_uint64(), ctypes.c_uint64()
+        lu, busy = ctypes.c_double(), ctypes.c_double()
+        self._check(lib().lgf_server_stats(self.h, ctypes.byref(jd), ctypes.byref(bs), ctypes.byref(lu),
+                                           ctypes.byref(busy)), "stats")
+        return {"jobs_done": jd.value, "batch_steps": bs.value, "link_updates": lu.value, "busy_s": busy.value}
+
+    def close(self):
+        if self.h:
+            lib().lgf_server_destroy(self.h)
             self.h = None
 
     def __del__(self):

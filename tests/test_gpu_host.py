@@ -1,0 +1,126 @@
+"""The C++ host program (msc_project_amd/lgf) as a drop-in for the reference's
+FlowSavedGaugeConfigurations / MeasureQdensityDistribution: same positional
+CLI, same directory layout and file names, CSVs identical to the reference's
+6-digit files, full-precision twins within the FP64 parity bar, q(x) in the
+reference's pi^2/4 scale."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from tests.common import GOLDEN, TOL_FP64, config1_inputs, golden, q_err, rel_err, small_input
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LGF = os.path.join(ROOT, "msc_project_amd", "lgf")
+
+
+def beta_dir(beta):
+    return "beta" + ("%f" % beta).replace(".", "_")
+
+
+def place_inputs(folder, shape, beta, cfgs):
+    d = os.path.join(folder, "ensembles", "su3", beta_dir(beta), "X".join(map(str, shape)), "LHMC_3_of_0_330000")
+    os.makedirs(d, exist_ok=True)
+    for i, U in enumerate(cfgs):
+        U.tofile(os.path.join(d, "torus_extdof4_%d.bin" % i))
+
+
+def run_lgf(folder, shape, beta, c1, steps, eps, every, *opts):
+    args = [LGF, "4", "4", repr(beta), *map(str, shape), "0", "0", "0", "0", "0", str(c1), str(steps), repr(eps),
+            str(every), folder, *opts]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=600)
+    assert out.returncode == 0, out.stderr
+    return out.stdout
+
+
+def read_full(path):
+    rows = [l.split(",") for l in open(path).read().split()]
+    return np.array([float(r[0]) for r in rows]), np.array([float(r[1]) for r in rows])
+
+
+def csv_equal_or_rounding_tie(text, ref_text, full_vals):
+    if text == ref_text:
+        return True
+    # 6-significant-digit text may differ only where the value sits at a rounding tie
+    for a, b, v in zip(text.split(), ref_text.split(), full_vals):
+        if a != b and abs(float(a.split(",")[1]) - float(b.split(",")[1])) > 1.01e-6 * max(abs(v), 1e-300) * 10:
+            return False
+    return True
+
+
+@pytest.mark.parametrize("mode", ["flow", "batch"])
+def test_host_flow_config1(tmp_path, mode):
+    d, shape, cfgs = config1_inputs()
+    folder = str(tmp_path) + "/"
+    place_inputs(folder, shape, d["beta"], cfgs)
+    out = run_lgf(folder, shape, d["beta"], len(cfgs) - 1, d["steps"], d["eps"], d["every"], "--mode=" + mode)
+    # console lines in the reference's order: per configuration, Q then E at each measurement
+    lines = [l for l in out.splitlines() if l.startswith("Topological Charge") or l.startswith("Energy Density")]
+    assert len(lines) == 2 * len(cfgs) * (1 + d["steps"] // d["every"])
+    assert lines[0].startswith("Topological Charge") and lines[1].startswith("Energy Density")
+    obs = os.path.join(folder, "Observables")
+    sd = "X".join(map(str, shape))
+    for i, c in enumerate(d["configs"]):
+        qdir = os.path.join(obs, "Topological_Charge", beta_dir(d["beta"]), sd, "GF")
+        edir = os.path.join(obs, "EnergyDensity", beta_dir(d["beta"]), sd, "GF")
+        tq, Q = read_full(os.path.join(qdir, "torus_extdof4_%d.full.csv" % i))
+        te, E = read_full(os.path.join(edir, "torus_extdof4_%d.full.csv" % i))
+        assert rel_err(E, c["E"]) < TOL_FP64
+        assert np.max(np.abs(Q - np.asarray(c["Q"]))) < TOL_FP64 * max(1.0, np.max(np.abs(c["Q"])))
+        assert csv_equal_or_rounding_tie(open(os.path.join(qdir, "torus_extdof4_%d.csv" % i)).read(), c["csv_Q"], Q)
+        assert csv_equal_or_rounding_tie(open(os.path.join(edir, "torus_extdof4_%d.csv" % i)).read(), c["csv_E"], E)
+        flowed = os.path.join(folder, "ensembles", "su3", beta_dir(d["beta"]), sd, "GF", c["flowed_name"])
+        assert os.path.getsize(flowed) == np.prod(shape) * 576
+
+
+def test_host_qdensity_small(tmp_path):
+    d, U = small_input()
+    shape = tuple(d["shape"])
+    c = d["configs"][0]
+    folder = str(tmp_path) + "/"
+    place_inputs(folder, shape, d["beta"], [U])
+    run_lgf(folder, shape, d["beta"], 0, d["steps"], d["eps"], d["every"])
+    run_lgf(folder, shape, d["beta"], 0, d["steps"], d["eps"], d["every"], "--mode=qdensity")
+    qx = os.path.join(folder, "Observables", "Qdensity", beta_dir(d["beta"]), "X".join(map(str, shape)), "GF",
+                      c["qx_name"])
+    q = np.fromfile(qx, dtype=np.float64)
+    assert q.size == np.prod(shape)
+    assert abs(q.sum() - c["qx_sum"]) < TOL_FP64 * c["qx_abs_sum"]
+    for k, v in c["qx_samples"].items():
+        assert abs(q[int(k)] - v) < TOL_FP64 * c["qx_abs_sum"]
+    # the flowed configuration written by the host equals the reference's at its first/last link
+    fl = np.fromfile(os.path.join(folder, "ensembles", "su3", beta_dir(d["beta"]), "X".join(map(str, shape)), "GF",
+                                  c["flowed_name"]))
+    assert np.max(np.abs(fl[:18] - c["flowed_first_link"])) < 1e-13
+    assert np.max(np.abs(fl[-18:] - c["flowed_last_link"])) < 1e-13
+
+
+def test_host_resume_continues_flow(tmp_path):
+    """ResumeFlowedConfiguration: 10 steps + resume 10 steps == 20 steps at the end"""
+    d, U = small_input()
+    shape = tuple(d["shape"])
+    a, b = str(tmp_path / "a") + "/", str(tmp_path / "b") + "/"
+    place_inputs(a, shape, d["beta"], [U])
+    place_inputs(b, shape, d["beta"], [U])
+    run_lgf(a, shape, d["beta"], 0, 20, 0.02, 5)
+    run_lgf(b, shape, d["beta"], 0, 10, 0.02, 5)
+    run_lgf(b, shape, d["beta"], 0, 10, 0.02, 5, "--mode=resume", "--pickup=%r" % (10 * 0.02))
+    sd = "X".join(map(str, shape))
+    ta, Qa = read_full(os.path.join(a, "Observables", "Topological_Charge", beta_dir(d["beta"]), sd, "GF",
+                                    "torus_extdof4_0.full.csv"))
+    tb, Qb = read_full(os.path.join(b, "Observables", "Topological_Charge", beta_dir(d["beta"]), sd, "GF",
+                                    "torus_extdof4_0.full.csv"))
+    # resume does not measure at the pickup time (LGF/LGF.cpp:197-226) and appends to the same CSV:
+    # rows t = 0, 0.1, 0.2 from the first run, then 0.3, 0.4 -- the same rows as one 20-step run
+    assert len(Qb) == len(Qa)
+    assert np.allclose(tb, ta, rtol=0, atol=1e-12)
+    assert np.array_equal(Qb, Qa)  # the saved configuration is exact in FP64, so the continuation is too
+
+
+def test_host_missing_input_errors(tmp_path):
+    out = subprocess.run([LGF, "4", "4", "6.0", "4", "4", "4", "4", "0", "0", "0", "0", "0", "0", "5", "0.02", "5",
+                          str(tmp_path) + "/"], capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0 and "cannot open" in out.stderr
