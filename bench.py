@@ -66,10 +66,13 @@ def make_input(seed=1234):
 
 
 def run_path(lgf, ctx, U, prec, steps, warmup, ev):
+    variant, tune_ms = ctx.autotune(SHAPE, prec, 6)
     f = lgf.Field(ctx, SHAPE, prec)
     f.upload(U)
     b = lgf.Batch(ctx, [f])
-    flags = lgf.MEASURE_FP64
+    # intermediate measurements: clover arithmetic in the path's precision with
+    # FP64 reductions; the final sweep below is always FP64
+    flags = lgf.MEASURE_DEFAULT
     # warmup (also captures the graphs)
     for s in range(warmup):
         b.flow(EPS, 1, use_graph=True)
@@ -102,12 +105,15 @@ def run_path(lgf, ctx, U, prec, steps, warmup, ev):
             b.measure(flags)
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    E, Q, Qa = b.measure(flags)
+    E, Q, Qa = b.measure(lgf.MEASURE_FP64)
+    E_native, Q_native, _ = b.measure(flags)
     diag = f.diagnostics()
     b.close()
     f.close()
     return {"wall_s": wall, "flow_ms_per_step": flow_ms, "measure_ms": meas_ms, "E": float(E[0]),
-            "Q": float(Q[0]), "unitarity_max": diag[1]}
+            "Q": float(Q[0]), "E_native": float(E_native[0]), "Q_native": float(Q_native[0]),
+            "unitarity_max": diag[1], "stage_variant": variant,
+            "autotune_ms_per_step": tune_ms}
 
 
 def cpu_baseline(U, budget_s=20.0):
@@ -131,13 +137,40 @@ def cpu_baseline(U, budget_s=20.0):
                       "OpenMP over sites), no measurements, %.1f s" % (n, dt)}
 
 
+def profile_run(which, steps):
+    """Fixed workload for rocprofv3 counter runs: `steps` RK3 steps of the
+    24^4 lattice in one precision (direct launches, default stage variant),
+    one clover measurement, and one plain copy of a link array (k_convert of
+    72*stride elements: a kernel of known traffic to calibrate the counters)."""
+    from msc_project_amd import lgf
+    ctx = lgf.Context(0)
+    prec = lgf.FP64 if which == "fp64" else lgf.FP32
+    f = lgf.Field(ctx, SHAPE, prec)
+    f.set_random(1234)
+    f.flow(EPS, steps)
+    f.measure(lgf.MEASURE_FP64 if which != "fp32native" else lgf.MEASURE_DEFAULT)
+    g = lgf.Field(ctx, SHAPE, prec)
+    g.copy_from(f)
+    ctx.synchronize()
+    stride = (np.prod(SHAPE) + 63) // 64 * 64
+    nbytes = 72 * stride * (8 if prec == lgf.FP64 else 4)
+    print(json.dumps({"profile": which, "steps": steps, "copy_kernel_read_bytes": int(nbytes),
+                      "copy_kernel_write_bytes": int(nbytes),
+                      "stage_algorithmic_bytes_per_step": BYTES_PER_LU[prec] * 4 * int(np.prod(SHAPE))}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--profile", choices=["fp32", "fp64", "fp32native"], default=None,
+                    help="run the fixed counter-profiling workload instead of the benchmark")
     args = ap.parse_args()
+    if args.profile:
+        profile_run(args.profile, args.steps)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the flow path runs one lattice (or one batch) per GPU "
               "and is not sharded across GPUs; run with --gpus 1", file=sys.stderr)
@@ -173,6 +206,7 @@ def main():
         "fp64_lups": lu_step * args.steps / r64["wall_s"],
         "fp64_flow_only_lups": lu_step / (r64["flow_ms_per_step"] * 1e-3),
         "fp32_over_fp64_flow": r64["flow_ms_per_step"] / r32["flow_ms_per_step"],
+        "stage_variant": {"fp64": r64["stage_variant"], "fp32": r32["stage_variant"]},
         "measure_ms": {"fp32_links": r32["measure_ms"], "fp64_links": r64["measure_ms"]},
         "fp32_vs_fp64_final": {"dE_rel": abs(r32["E"] - r64["E"]) / abs(r64["E"]), "dQ": abs(r32["Q"] - r64["Q"])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

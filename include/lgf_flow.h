@@ -82,10 +82,15 @@ const char* lgf_ctx_last_error(const lgf_ctx* ctx);
 /* the hipStream_t every call of this context is ordered on */
 int lgf_ctx_stream(lgf_ctx* ctx, void** hip_stream);
 int lgf_ctx_synchronize(lgf_ctx* ctx);
-/* block shape choice for the stage kernel: 0 = tuned default; see lgf_autotune */
+/* RK3 stage-kernel variant (register budget x workgroup shape) used by this
+ * context for one precision; 0 = default */
 int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
 int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
 int lgf_num_stage_variants(int precision, int* n);
+/* time every variant for nsteps RK3 steps on a seeded lattice of `shape`,
+ * select the fastest for this context; ms_per_step (optional) gets one entry
+ * per variant */
+int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best, double* ms_per_step);
 
 /* ---- one lattice ------------------------------------------------------ */
 /* device bytes held by a field of this shape and precision */

@@ -328,8 +328,8 @@ extern "C" int lgf_field_copy(lgf_field* dst, const lgf_field* src) {
     lgf_ctx* ctx = dst->ctx;
     for (int i = 0; i < 4; ++i)
         if (dst->shape[i] != src->shape[i]) return lgf_fail(ctx, LGF_E_SHAPE, "copy between different shapes");
-    HIPTRY(ctx, launch_convert(src->prec, src->links[src->cur], dst->prec, dst->links[dst->cur],
-                               int64_t(72) * dst->stride, ctx->stream));
+    HIPTRY(ctx, launch_convert(src->prec, src->links[src->cur], dst->prec, dst->links[dst->cur], dst->stride,
+                               dst->vol, ctx->stream));
     return LGF_OK;
 }
 
@@ -567,4 +567,53 @@ extern "C" int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, 
     }
     if (!finite) return lgf_fail(ctx, LGF_E_NAN, "non-finite energy density or charge");
     return LGF_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner
+// ---------------------------------------------------------------------------
+// Times every stage-kernel variant on a seeded random lattice of the given
+// shape (nsteps RK3 steps each, after one warm-up step, HIP events on the
+// context's stream) and makes the fastest the context's variant for that
+// precision.  ms_per_step (optional) receives one time per variant.
+extern "C" int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best,
+                            double* ms_per_step) {
+    if (!ctx || nsteps <= 0 || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
+    lgf_field* f = nullptr;
+    int rc = lgf_field_create(ctx, shape, precision, &f);
+    if (rc) return rc;
+    rc = lgf_field_set_random(f, 0x5eedULL);
+    lgf_batch* b = nullptr;
+    if (!rc) rc = lgf_batch_create(ctx, &f, 1, &b);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = lgf_fail(ctx, LGF_E_HIP, "event creation");
+    const int nv = num_stage_variants(precision);
+    int arg = 0;
+    double best_ms = 1e300;
+    const double eps = 0.01;
+    for (int v = 0; v < nv && !rc; ++v) {
+        ctx->stage_variant[precision] = v;
+        rc = lgf_batch_flow_steps(b, &eps, 1, 0);
+        if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = LGF_E_HIP;
+        if (!rc) rc = lgf_batch_flow_steps(b, &eps, nsteps, 0);
+        if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LGF_E_HIP;
+        float ms = 0.f;
+        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+            rc = lgf_fail(ctx, LGF_E_HIP, "event timing");
+        if (rc) break;
+        const double per = (double)ms / nsteps;
+        if (ms_per_step) ms_per_step[v] = per;
+        if (per < best_ms) {
+            best_ms = per;
+            arg = v;
+        }
+    }
+    ctx->stage_variant[precision] = arg;
+    if (best) *best = arg;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (b) lgf_batch_destroy(b);
+    lgf_field_destroy(f);
+    return rc;
 }

@@ -12,8 +12,8 @@
 //   Q^{n+1} = c0 d_n + (a_n + c1 d_n) Q + b_n Q^2 for Q^n = a_n + b_n Q + d_n Q^2.
 //   No transcendental functions; NTERMS = 13 gives a truncation error below
 //   |lambda|^13/13! (< 2e-10 for |lambda| <= 1, where lambda are the
-//   eigenvalues of Q).  Links whose exponent is larger than that fall back to
-//   the closed form in double precision.  Used by the mixed-precision path.
+//   eigenvalues of Q).  Larger exponents are scaled and squared.  Used by
+//   the mixed-precision path.
 #pragma once
 #include "su3.hpp"
 
@@ -90,6 +90,9 @@ LGF_DEV M3<double> exp_ah_cayley(const M3<double>& X) {
 }
 
 // Power series through the Cayley-Hamilton recurrence, real type R.
+// Valid for c1 <= 0.75 (all |lambda| <= 1); larger exponents (rare: a flow
+// step moves a link by eps*|Z|) are scaled by 2^-k into that range and the
+// result squared k times, all in precision R.
 template <typename R, int NTERMS = 13>
 LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
     M3<R> Q;
@@ -99,18 +102,21 @@ LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
         Q.v[2 * i + 1] = -X.v[2 * i];
     }
     M3<R> Q2 = mul_nn(Q, Q);
-    R c0 = retr_mul(Q, Q2) * (R(1) / R(3));
     R c1 = retr(Q2) * R(0.5);
-    if (c1 > R(0.75)) {  // |lambda| may exceed 1: closed form in double
-        M3<double> Xd;
+    int k = 0;
+    if (c1 > R(0.75)) {  // scaling: c1 ~ |Q|^2 shrinks by 4 per halving
+        R scale = R(1);
+        while (c1 > R(0.75) && k < 30) {
+            c1 *= R(0.25);
+            scale *= R(0.5);
+            ++k;
+        }
 #pragma unroll
-        for (int i = 0; i < 18; ++i) Xd.v[i] = (double)X.v[i];
-        M3<double> Ed = exp_ah_cayley(Xd);
-        M3<R> E;
-#pragma unroll
-        for (int i = 0; i < 18; ++i) E.v[i] = (R)Ed.v[i];
-        return E;
+        for (int i = 0; i < 18; ++i) Q.v[i] *= scale;
+        Q2 = mul_nn(Q, Q);
+        c1 = retr(Q2) * R(0.5);
     }
+    R c0 = retr_mul(Q, Q2) * (R(1) / R(3));
     // f_j = sum_n i^n/n! * coef_j(Q^n); real part from n = 0,2 (mod 4) with
     // signs +,-, imaginary part from n = 1,3 (mod 4) with signs +,-.
     R a = R(1), b = R(0), d = R(0);
@@ -134,8 +140,8 @@ LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         R qr = Q.v[2 * i], qi = Q.v[2 * i + 1], pr = Q2.v[2 * i], pi = Q2.v[2 * i + 1];
-        o.v[2 * i] = (qr * f1r - qi * f1i) + (pr * f2r - pi * f2i);
-        o.v[2 * i + 1] = (qr * f1i + qi * f1r) + (pr * f2i + pi * f2r);
+        o.v[2 * i] = qr * f1r - qi * f1i + pr * f2r - pi * f2i;
+        o.v[2 * i + 1] = qr * f1i + qi * f1r + pr * f2i + pi * f2r;
     }
     o.v[0] += f0r;
     o.v[1] += f0i;
@@ -143,6 +149,7 @@ LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
     o.v[9] += f0i;
     o.v[16] += f0r;
     o.v[17] += f0i;
+    for (int s = 0; s < k; ++s) o = mul_nn(o, o);
     return o;
 }
 

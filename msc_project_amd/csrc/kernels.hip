@@ -32,14 +32,17 @@ namespace lgf {
 // ---------------------------------------------------------------------------
 // RK3 stage
 // ---------------------------------------------------------------------------
-template <typename S, typename C, int STAGE, int EXPMODE, int WPE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage(const FieldDesc* __restrict__ fds,
-                                                     const int2* __restrict__ tiles, int ping) {
+// BY = directions per workgroup: 4 (block 64x4, mu = threadIdx.y: the four
+// links of a site share their neighbours in L1/L2) or 1 (block 64, mu =
+// blockIdx.y).  WPE = minimum waves per SIMD the register budget must allow.
+template <typename S, typename C, int STAGE, int EXPMODE, int WPE, int BY>
+__global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage(
+    const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int ping) {
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int2 tm = tiles[t];
     const FieldDesc& d = fds[tm.x];
     const int s = tm.y * kTileSites + threadIdx.x;
-    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int mu = BY == 4 ? __builtin_amdgcn_readfirstlane(threadIdx.y) : (int)blockIdx.y;
     if (s >= d.vol) return;
     const int cur = ping ^ d.flip;
     const S* __restrict__ U = (const S*)d.links[cur];
@@ -62,14 +65,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             M3<C> b = load_link<S, C>(U, st, mu, s + fnu);
             M3<C> tt = mul_nn(a, b);
             M3<C> c = load_link<S, C>(U, st, nu, s + fmu);
-            add_to(A, mul_nd(c, tt));
+            mul_nd_acc(A, c, tt);
         }
         {
             M3<C> a = load_link<S, C>(U, st, mu, s + bnu);
             M3<C> b = load_link<S, C>(U, st, nu, s + bnu + fmu);
             M3<C> tt = mul_nn(a, b);
             M3<C> c = load_link<S, C>(U, st, nu, s + bnu);
-            add_to(A, mul_dn(tt, c));
+            mul_dn_acc(A, tt, c);
         }
     }
     const M3<C> Umu = load_link<S, C>(U, st, mu, s);
@@ -117,76 +120,119 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 // ---------------------------------------------------------------------------
 // clovers, E, Q, q(x)
 // ---------------------------------------------------------------------------
-// Path products.  A step along +d multiplies by U_d(here) and moves to
-// here+d; a step along -d moves to here-d and multiplies by U_d(here)^dag.
-// The first link of a path initialises the product.
-template <typename S, typename C>
-LGF_DEV void path_first(const S* __restrict__ U, int64_t st, Geo& g, M3<C>& P, int dir, int sgn) {
-    if (sgn > 0) {
-        P = load_link<S, C>(U, st, dir, g.s);
-        geo_move(g, dir, +1);
-    } else {
-        geo_move(g, dir, -1);
-        const M3<C> L = load_link<S, C>(U, st, dir, g.s);
-#pragma unroll
-        for (int r = 0; r < 3; ++r)
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                P.v[2 * (3 * r + c)] = L.v[2 * (3 * c + r)];
-                P.v[2 * (3 * r + c) + 1] = -L.v[2 * (3 * c + r) + 1];
-            }
-    }
+// Clover leaves.
+//
+// A leaf of extent (la x lb) in the quadrant spanned by the signed
+// directions a, b starts at x, takes la steps along a, lb along b, la back
+// along -a and lb back along -b.  A step along +d multiplies by U_d(here),
+// a step along -d by U_d(here - d)^dag.  The four quadrants of the (mu,nu)
+// plane are those of the reference's leaves, (a,b) in
+// {(+nu,+mu), (+mu,-nu), (-mu,+nu), (-nu,-mu)}; e.g. the first 1x1 leaf is
+// U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag U_mu(x)^dag (LGF/Field/SU3_field.cpp:
+// 550-560), and the 2x2, 1x2 and 1x3 clovers (LGF/Field/SU3_field.cpp:
+// 598-792) are the same four leaves with longer legs.
+//
+// Every link of a leaf is an "a-link" (along a's axis) between a-positions i
+// and i+1 at b-position j, or a "b-link".  Their site offsets are computed
+// once per quadrant; the walk itself is unrolled at compile time, and the
+// dagger of a step is folded into which components are loaded (a uniform,
+// scalar choice) and the sign of the imaginary parts.
+struct Quadrant {
+    int off_a[4], off_b[4];  // site offset of i steps along a (j along b)
+    int lnk_a[3], lnk_b[3];  // offset of the site holding the i-th a-link (j-th b-link)
+    int dir_a, dir_b;        // axes of a and b
+    bool neg_a, neg_b;       // a (b) points against its axis
+};
+
+LGF_DEV int wrap_offset(const Geo& g, int dir, int steps) {
+    const int xd = geo_x(g, dir), nd = geo_n(g, dir);
+    int v = (xd + steps) % nd;
+    v = v < 0 ? v + nd : v;
+    return (v - xd) * geo_str(g, dir);
 }
 
-template <typename S, typename C>
-LGF_DEV void path_step(const S* __restrict__ U, int64_t st, Geo& g, M3<C>& P, int dir, int sgn) {
-    if (sgn > 0) {
-        const M3<C> L = load_link<S, C>(U, st, dir, g.s);
-        P = mul_nn(P, L);
-        geo_move(g, dir, +1);
-    } else {
-        geo_move(g, dir, -1);
-        const M3<C> L = load_link<S, C>(U, st, dir, g.s);
-        P = mul_nd(P, L);
+LGF_DEV Quadrant make_quadrant(const Geo& g, int dir_a, bool neg_a, int dir_b, bool neg_b) {
+    Quadrant q;
+    q.dir_a = dir_a;
+    q.dir_b = dir_b;
+    q.neg_a = neg_a;
+    q.neg_b = neg_b;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        q.off_a[i] = wrap_offset(g, dir_a, neg_a ? -i : i);
+        q.off_b[i] = wrap_offset(g, dir_b, neg_b ? -i : i);
     }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        q.lnk_a[i] = neg_a ? q.off_a[i + 1] : q.off_a[i];
+        q.lnk_b[i] = neg_b ? q.off_b[i + 1] : q.off_b[i];
+    }
+    return q;
 }
 
-// Sum of the four leaves of extent (lmu x lnu) in the (mu,nu) plane,
-// weighted by w, added to acc (and, unweighted, to acc2 when ALSO_PLAIN).
-// The leaves are oriented as the reference's (first leg along a, second
-// along b, (a,b) in {(+nu,+mu), (+mu,-nu), (-mu,+nu), (-nu,-mu)}), e.g. the
-// first 1x1 leaf is U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag U_mu(x)^dag
-// (LGF/Field/SU3_field.cpp:550-560); the 2x2, 1x2 and 1x3 clovers of
-// LGF/Field/SU3_field.cpp:598-792 are the same four leaves with longer legs.
-template <typename S, typename C, bool ALSO_PLAIN>
-LGF_DEV void add_leaves(const S* __restrict__ U, int64_t st, const Geo& g0, int mu, int nu, int lmu,
-                        int lnu, C w, M3<C>& acc, M3<C>& acc2) {
-#pragma unroll 1
-    for (int q = 0; q < 4; ++q) {
-        const int adir = (q == 0 || q == 3) ? nu : mu;
-        const int asg = (q == 2 || q == 3) ? -1 : +1;
-        const int bdir = (q == 0 || q == 3) ? mu : nu;
-        const int bsg = (q == 1 || q == 3) ? -1 : +1;
-        const int la = (adir == mu) ? lmu : lnu;
-        const int lb = (bdir == mu) ? lmu : lnu;
-        Geo g = g0;
-        M3<C> P;
-        path_first<S, C>(U, st, g, P, adir, asg);
-#pragma unroll 1
-        for (int i = 1; i < la; ++i) path_step<S, C>(U, st, g, P, adir, asg);
-#pragma unroll 1
-        for (int i = 0; i < lb; ++i) path_step<S, C>(U, st, g, P, bdir, bsg);
-#pragma unroll 1
-        for (int i = 0; i < la; ++i) path_step<S, C>(U, st, g, P, adir, -asg);
-#pragma unroll 1
-        for (int i = 0; i < lb; ++i) path_step<S, C>(U, st, g, P, bdir, -bsg);
+// U_dir at site s, or its conjugate transpose when dag (uniform): a
+// branch-free select per element, so the matrix stays in registers
+template <typename S, typename C>
+LGF_DEV M3<C> load_link_maybe_dag(const S* __restrict__ U, int64_t st, int dir, int s, bool dag) {
+    const M3<C> m = load_link<S, C>(U, st, dir, s);
+    M3<C> t;
+    const C sgn = dag ? C(-1) : C(1);
 #pragma unroll
-        for (int i = 0; i < 18; ++i) acc.v[i] += w * P.v[i];
-        if constexpr (ALSO_PLAIN) {
+    for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int i = 0; i < 18; ++i) acc2.v[i] += P.v[i];
+        for (int c = 0; c < 3; ++c) {
+            const int a = 2 * (3 * r + c), b = 2 * (3 * c + r);
+            t.v[a] = (r == c) ? m.v[a] : (dag ? m.v[b] : m.v[a]);
+            t.v[a + 1] = sgn * ((r == c) ? m.v[a + 1] : (dag ? m.v[b + 1] : m.v[a + 1]));
         }
+    return t;
+}
+
+// Step k of the leaf walk (compile-time k after unrolling): axis, site and
+// whether the link is traversed backwards (daggered).
+template <int LA, int LB>
+LGF_DEV void leaf_link(int k, int s, const Quadrant& q, int& dir, int& site, bool& dag) {
+    if (k < LA) {  // out along a, row j = 0
+        dir = q.dir_a, site = s + q.lnk_a[k] + q.off_b[0], dag = q.neg_a;
+    } else if (k < LA + LB) {  // out along b, column i = LA
+        dir = q.dir_b, site = s + q.off_a[LA] + q.lnk_b[k - LA], dag = q.neg_b;
+    } else if (k < 2 * LA + LB) {  // back along -a, row j = LB
+        dir = q.dir_a, site = s + q.lnk_a[2 * LA + LB - 1 - k] + q.off_b[LB], dag = !q.neg_a;
+    } else {  // back along -b, column i = 0
+        dir = q.dir_b, site = s + q.off_a[0] + q.lnk_b[2 * LA + 2 * LB - 1 - k], dag = !q.neg_b;
     }
+}
+
+// Product of the 2(LA+LB) links of a leaf.  The next link is always in
+// flight while the current product is formed; the empty asm with a memory
+// clobber keeps the compiler from hoisting loads further ahead, which would
+// hold whole leaves in registers.
+template <typename S, typename C, int LA, int LB>
+LGF_DEV M3<C> leaf(const S* __restrict__ U, int64_t st, int s, const Quadrant& q) {
+    constexpr int N = 2 * (LA + LB);
+    int dir, site;
+    bool dag;
+    leaf_link<LA, LB>(0, s, q, dir, site, dag);
+    M3<C> P = load_link_maybe_dag<S, C>(U, st, dir, site, dag);
+    leaf_link<LA, LB>(1, s, q, dir, site, dag);
+    M3<C> Lnext = load_link_maybe_dag<S, C>(U, st, dir, site, dag);
+#pragma unroll
+    for (int k = 1; k < N; ++k) {
+        const M3<C> L = Lnext;
+        if (k + 1 < N) {
+            leaf_link<LA, LB>(k + 1, s, q, dir, site, dag);
+            Lnext = load_link_maybe_dag<S, C>(U, st, dir, site, dag);
+        }
+        asm volatile("" ::: "memory");
+        P = mul_nn(P, L);
+    }
+    return P;
+}
+
+template <typename C>
+LGF_DEV void axpy(M3<C>& acc, C w, const M3<C>& P) {
+#pragma unroll
+    for (int i = 0; i < 18; ++i) acc.v[i] += w * P.v[i];
 }
 
 // F = 1/2 (-i) [ (C - C^dag) - 1/3 Tr(C - C^dag) ] (LGF/Field/SU3_field.cpp:575-579)
@@ -210,12 +256,23 @@ LGF_DEV void plane_F(const S* __restrict__ U, int64_t st, const Geo& g, int mu, 
     set_zero(Ci);
     set_zero(C11);
     const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
-    add_leaves<S, C, true>(U, st, g, mu, nu, 1, 1, C(0.25) * k1, Ci, C11);
-    add_leaves<S, C, false>(U, st, g, mu, nu, 2, 2, C(0.25) * k2, Ci, C11);
-    add_leaves<S, C, false>(U, st, g, mu, nu, 1, 2, C(0.125) * k3, Ci, C11);
-    add_leaves<S, C, false>(U, st, g, mu, nu, 2, 1, C(0.125) * k3, Ci, C11);
-    add_leaves<S, C, false>(U, st, g, mu, nu, 1, 3, C(0.125) * k4, Ci, C11);
-    add_leaves<S, C, false>(U, st, g, mu, nu, 3, 1, C(0.125) * k4, Ci, C11);
+#pragma unroll 1
+    for (int qd = 0; qd < 4; ++qd) {
+        // (a,b) = (+nu,+mu), (+mu,-nu), (-mu,+nu), (-nu,-mu)
+        const int dir_a = (qd == 0 || qd == 3) ? nu : mu;
+        const int dir_b = (qd == 0 || qd == 3) ? mu : nu;
+        const bool neg_a = qd >= 2, neg_b = (qd & 1) != 0;
+        const Quadrant q = make_quadrant(g, dir_a, neg_a, dir_b, neg_b);
+        // (la, lb) = extents along (a, b); both orientations of each
+        // rectangle enter with the same weight
+        add_to(C11, leaf<S, C, 1, 1>(U, st, g.s, q));
+        axpy(Ci, C(0.25) * k2, leaf<S, C, 2, 2>(U, st, g.s, q));
+        axpy(Ci, C(0.125) * k3, leaf<S, C, 1, 2>(U, st, g.s, q));
+        axpy(Ci, C(0.125) * k3, leaf<S, C, 2, 1>(U, st, g.s, q));
+        axpy(Ci, C(0.125) * k4, leaf<S, C, 1, 3>(U, st, g.s, q));
+        axpy(Ci, C(0.125) * k4, leaf<S, C, 3, 1>(U, st, g.s, q));
+    }
+    axpy(Ci, C(0.25) * k1, C11);
 #pragma unroll
     for (int i = 0; i < 18; ++i) C11.v[i] *= C(0.25);
     Fimp = project_F(Ci);
@@ -400,31 +457,35 @@ __global__ __launch_bounds__(256) void k_reduce_max(const FieldDesc* __restrict_
 // ---------------------------------------------------------------------------
 // layout conversion and initialisation
 // ---------------------------------------------------------------------------
-// file order (site-major, 4 links x 18 doubles) for sites [s0, s0+n) -> SoA
+// file order (site-major, 4 links x 18 doubles) for sites [s0, s0+n) -> device layout
 template <typename S>
 __global__ __launch_bounds__(256) void k_fileorder_to_soa(const double* __restrict__ src, S* __restrict__ dst,
                                                           int64_t stride, int s0, int n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // element within chunk
-    if (i >= (int64_t)n * 72) return;
-    const int ls = (int)(i / 72), r = (int)(i % 72);
-    const int mu = r / 18, c = r % 18;
-    dst[((int64_t)mu * 18 + c) * stride + s0 + ls] = (S)src[i];
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;  // link within chunk
+    if (l >= (int64_t)n * 4) return;
+    M3<double> m;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) m.v[c] = src[l * 18 + c];
+    store_link<S, double>(dst, stride, (int)(l & 3), s0 + (int)(l >> 2), m);
 }
 
 template <typename S>
 __global__ __launch_bounds__(256) void k_soa_to_fileorder(const S* __restrict__ src, double* __restrict__ dst,
                                                           int64_t stride, int s0, int n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)n * 72) return;
-    const int ls = (int)(i / 72), r = (int)(i % 72);
-    const int mu = r / 18, c = r % 18;
-    dst[i] = (double)src[((int64_t)mu * 18 + c) * stride + s0 + ls];
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (int64_t)n * 4) return;
+    const M3<double> m = load_link<S, double>(src, stride, (int)(l & 3), s0 + (int)(l >> 2));
+#pragma unroll
+    for (int c = 0; c < 18; ++c) dst[l * 18 + c] = m.v[c];
 }
 
 template <typename SI, typename SO>
-__global__ __launch_bounds__(256) void k_convert(const SI* __restrict__ src, SO* __restrict__ dst, int64_t n) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) dst[i] = (SO)src[i];
+__global__ __launch_bounds__(256) void k_convert(const SI* __restrict__ src, SO* __restrict__ dst, int64_t stride,
+                                                 int vol) {
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (int64_t)vol * 4) return;
+    const int s = (int)(l >> 2), mu = (int)(l & 3);
+    store_link<SO, double>(dst, stride, mu, s, load_link<SI, double>(src, stride, mu, s));
 }
 
 // Seeded Haar-random SU(3) links: rows 0 and 1 from complex Gaussians
@@ -450,48 +511,51 @@ __global__ __launch_bounds__(256) void k_hot(S* __restrict__ dst, int64_t stride
 
 template <typename S>
 __global__ __launch_bounds__(256) void k_cold(S* __restrict__ dst, int64_t stride, int vol) {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (int64_t)72 * stride) return;
-    const int c = (int)((i / stride) % 18);
-    const int64_t s = i % stride;
-    (void)vol;
-    dst[i] = (S)((c == 0 || c == 8 || c == 16) && s < vol ? 1.0 : 0.0);
+    const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= (int64_t)stride * 4) return;  // padding sites too: every slot holds a valid number
+    M3<double> m;
+    set_unit(m);
+    store_link<S, double>(dst, stride, (int)(l & 3), (int)(l >> 2), m);
 }
 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename S, typename C, int EXPMODE, int WPE>
+template <typename S, typename C, int EXPMODE, int WPE, int BY>
 static hipError_t stage_launch(int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
                                hipStream_t stream) {
-    dim3 grid(ntiles), block(kTileSites, 4);
+    dim3 grid(ntiles, 4 / BY), block(kTileSites, BY);
     if (stage == 0)
-        hipLaunchKernelGGL((k_rk3_stage<S, C, 0, EXPMODE, WPE>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 0, EXPMODE, WPE, BY>), grid, block, 0, stream, fds, tiles, ping);
     else if (stage == 1)
-        hipLaunchKernelGGL((k_rk3_stage<S, C, 1, EXPMODE, WPE>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 1, EXPMODE, WPE, BY>), grid, block, 0, stream, fds, tiles, ping);
     else
-        hipLaunchKernelGGL((k_rk3_stage<S, C, 2, EXPMODE, WPE>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 2, EXPMODE, WPE, BY>), grid, block, 0, stream, fds, tiles, ping);
     return hipGetLastError();
 }
 
-// Stage-kernel variants differ in the register budget (minimum waves per
-// SIMD) the compiler is held to; variant 0 is the default.
-int num_stage_variants(int prec) { return prec == kPrecFP64 ? 3 : 4; }
+// Stage-kernel variants: (register budget as minimum waves per SIMD, block
+// shape).  Variant 0 is the default; lgf_autotune picks the fastest.
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 6; }
 
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream) {
     if (prec == kPrecFP64) {
         switch (variant) {
-            case 1: return stage_launch<double, double, 0, 2>(stage, fds, tiles, ntiles, ping, stream);
-            case 2: return stage_launch<double, double, 0, 4>(stage, fds, tiles, ntiles, ping, stream);
-            default: return stage_launch<double, double, 0, 3>(stage, fds, tiles, ntiles, ping, stream);
+            case 1: return stage_launch<double, double, 0, 2, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 2: return stage_launch<double, double, 0, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 3: return stage_launch<double, double, 0, 3, 1>(stage, fds, tiles, ntiles, ping, stream);
+            case 4: return stage_launch<double, double, 0, 2, 1>(stage, fds, tiles, ntiles, ping, stream);
+            default: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
         }
     }
     switch (variant) {
-        case 1: return stage_launch<float, float, 1, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 2: return stage_launch<float, float, 1, 5>(stage, fds, tiles, ntiles, ping, stream);
-        case 3: return stage_launch<float, float, 1, 2>(stage, fds, tiles, ntiles, ping, stream);
-        default: return stage_launch<float, float, 1, 3>(stage, fds, tiles, ntiles, ping, stream);
+        case 1: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 2: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 3: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 4: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
+        case 5: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
+        default: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
     }
 }
 
@@ -530,7 +594,7 @@ hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int nt
 }
 
 hipError_t launch_to_soa(int prec, const double* src, void* dst, int64_t stride, int s0, int n, hipStream_t stream) {
-    int64_t total = (int64_t)n * 72;
+    int64_t total = (int64_t)n * 4;
     dim3 grid((unsigned)((total + 255) / 256)), block(256);
     if (prec == kPrecFP64)
         hipLaunchKernelGGL(k_fileorder_to_soa<double>, grid, block, 0, stream, src, (double*)dst, stride, s0, n);
@@ -541,7 +605,7 @@ hipError_t launch_to_soa(int prec, const double* src, void* dst, int64_t stride,
 
 hipError_t launch_from_soa(int prec, const void* src, double* dst, int64_t stride, int s0, int n,
                            hipStream_t stream) {
-    int64_t total = (int64_t)n * 72;
+    int64_t total = (int64_t)n * 4;
     dim3 grid((unsigned)((total + 255) / 256)), block(256);
     if (prec == kPrecFP64)
         hipLaunchKernelGGL(k_soa_to_fileorder<double>, grid, block, 0, stream, (const double*)src, dst, stride, s0, n);
@@ -550,16 +614,21 @@ hipError_t launch_from_soa(int prec, const void* src, double* dst, int64_t strid
     return hipGetLastError();
 }
 
-hipError_t launch_convert(int prec_in, const void* src, int prec_out, void* dst, int64_t n, hipStream_t stream) {
-    dim3 grid((unsigned)((n + 255) / 256)), block(256);
+hipError_t launch_convert(int prec_in, const void* src, int prec_out, void* dst, int64_t stride, int vol,
+                          hipStream_t stream) {
+    dim3 grid((unsigned)(((int64_t)vol * 4 + 255) / 256)), block(256);
     if (prec_in == kPrecFP64 && prec_out == kPrecFP64)
-        hipLaunchKernelGGL((k_convert<double, double>), grid, block, 0, stream, (const double*)src, (double*)dst, n);
+        hipLaunchKernelGGL((k_convert<double, double>), grid, block, 0, stream, (const double*)src, (double*)dst,
+                           stride, vol);
     else if (prec_in == kPrecFP64)
-        hipLaunchKernelGGL((k_convert<double, float>), grid, block, 0, stream, (const double*)src, (float*)dst, n);
+        hipLaunchKernelGGL((k_convert<double, float>), grid, block, 0, stream, (const double*)src, (float*)dst,
+                           stride, vol);
     else if (prec_out == kPrecFP64)
-        hipLaunchKernelGGL((k_convert<float, double>), grid, block, 0, stream, (const float*)src, (double*)dst, n);
+        hipLaunchKernelGGL((k_convert<float, double>), grid, block, 0, stream, (const float*)src, (double*)dst,
+                           stride, vol);
     else
-        hipLaunchKernelGGL((k_convert<float, float>), grid, block, 0, stream, (const float*)src, (float*)dst, n);
+        hipLaunchKernelGGL((k_convert<float, float>), grid, block, 0, stream, (const float*)src, (float*)dst,
+                           stride, vol);
     return hipGetLastError();
 }
 
@@ -573,7 +642,7 @@ hipError_t launch_hot(int prec, void* dst, int64_t stride, int vol, uint64_t see
 }
 
 hipError_t launch_cold(int prec, void* dst, int64_t stride, int vol, hipStream_t stream) {
-    int64_t total = (int64_t)72 * stride;
+    int64_t total = (int64_t)4 * stride;
     dim3 grid((unsigned)((total + 255) / 256)), block(256);
     if (prec == kPrecFP64)
         hipLaunchKernelGGL(k_cold<double>, grid, block, 0, stream, (double*)dst, stride, vol);

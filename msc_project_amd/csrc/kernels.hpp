@@ -24,7 +24,8 @@ hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int nt
 hipError_t launch_to_soa(int prec, const double* src, void* dst, int64_t stride, int s0, int n, hipStream_t stream);
 hipError_t launch_from_soa(int prec, const void* src, double* dst, int64_t stride, int s0, int n,
                            hipStream_t stream);
-hipError_t launch_convert(int prec_in, const void* src, int prec_out, void* dst, int64_t n, hipStream_t stream);
+hipError_t launch_convert(int prec_in, const void* src, int prec_out, void* dst, int64_t stride, int vol,
+                          hipStream_t stream);
 hipError_t launch_hot(int prec, void* dst, int64_t stride, int vol, uint64_t seed, hipStream_t stream);
 hipError_t launch_cold(int prec, void* dst, int64_t stride, int vol, hipStream_t stream);
 

@@ -5,12 +5,16 @@
 // Lattice.cpp:423-436): s = ((x0*N1 + x1)*N2 + x2)*N3 + x3, last coordinate
 // fastest; periodic in every direction (the reference's "torus").
 //
-// Link storage is structure-of-arrays, one plane per (mu, real component):
-//   links[(mu*18 + c)*stride + s],  c = 2*(3*row+col) + {0: Re, 1: Im}
-// so that the 64 lanes of a wavefront, which hold 64 consecutive sites, read
-// each component with one coalesced 256/512-byte access.  `stride` is the
-// volume rounded up to a multiple of 64.  The accumulator of the low-storage
-// RK3 scheme uses the same layout.
+// Link storage is structure-of-arrays of short vectors.  For each direction
+// mu the 18 reals c = 2*(3*row+col) + {0: Re, 1: Im} of all sites form a
+// block of 18*stride elements, split into planes of vectors so that each
+// lane of a wavefront (64 consecutive sites) fetches a link with few wide
+// loads and the wavefront's access to a plane is one contiguous run:
+//   FP32: planes 0..3 hold float4 (components 4k..4k+3), plane 4 float2
+//         (components 16, 17): 5 loads per link;
+//   FP64: planes 0..8 hold double2 (one complex element each): 9 loads.
+// `stride` is the volume rounded up to a multiple of 64.  The accumulator of
+// the low-storage RK3 scheme uses the same layout.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -85,20 +89,80 @@ LGF_DEV void geo_move(Geo& g, int d, int sgn) {
     else g.x3 = v;
 }
 
+// Link arrays live in global memory; loads and stores go through
+// address-space-1 pointers built from a wavefront-uniform plane base (SGPRs)
+// plus the lane's 32-bit byte offset, which the compiler maps onto global
+// loads with a scalar base.
+template <typename S>
+using gptr = __attribute__((address_space(1))) S*;
+
+// native clang vector types (the HIP vector wrappers do not convert across
+// address spaces)
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef float vf2 __attribute__((ext_vector_type(2)));
+typedef double vd2 __attribute__((ext_vector_type(2)));
+
+template <typename V, typename S>
+LGF_DEV V ld(const S* plane, uint32_t byte_off) {
+    return *(const gptr<const V>)((const __attribute__((address_space(1))) char*)plane + byte_off);
+}
+template <typename V, typename S>
+LGF_DEV void st_(S* plane, uint32_t byte_off, const V& v) {
+    *(gptr<V>)((__attribute__((address_space(1))) char*)plane + byte_off) = v;
+}
+
+// element offset of component c of link (mu, s): the layout described above
+__host__ __device__ inline int64_t link_elem_offset(bool fp32, int64_t stride, int mu, int64_t s, int c) {
+    const int64_t b = (int64_t)mu * 18 * stride;
+    if (fp32) return c < 16 ? b + (int64_t)(c >> 2) * 4 * stride + s * 4 + (c & 3) : b + 16 * stride + s * 2 + (c - 16);
+    return b + (int64_t)(c >> 1) * 2 * stride + s * 2 + (c & 1);
+}
+
 template <typename S, typename C>
 LGF_DEV M3<C> load_link(const S* __restrict__ base, int64_t stride, int mu, int s) {
     M3<C> m;
-    const S* p = base + (int64_t)mu * 18 * stride + s;
+    const S* p = base + (int64_t)mu * 18 * stride;
+    if constexpr (sizeof(S) == 4) {
+        const uint32_t o4 = (uint32_t)s * 16u, o2 = (uint32_t)s * 8u;
 #pragma unroll
-    for (int c = 0; c < 18; ++c) m.v[c] = (C)p[c * stride];
+        for (int k = 0; k < 4; ++k) {
+            const vf4 v = ld<vf4>(p + (int64_t)k * 4 * stride, o4);
+            m.v[4 * k] = (C)v.x;
+            m.v[4 * k + 1] = (C)v.y;
+            m.v[4 * k + 2] = (C)v.z;
+            m.v[4 * k + 3] = (C)v.w;
+        }
+        const vf2 w = ld<vf2>(p + 16 * stride, o2);
+        m.v[16] = (C)w.x;
+        m.v[17] = (C)w.y;
+    } else {
+        const uint32_t o = (uint32_t)s * 16u;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            const vd2 v = ld<vd2>(p + (int64_t)k * 2 * stride, o);
+            m.v[2 * k] = (C)v.x;
+            m.v[2 * k + 1] = (C)v.y;
+        }
+    }
     return m;
 }
 
 template <typename S, typename C>
 LGF_DEV void store_link(S* __restrict__ base, int64_t stride, int mu, int s, const M3<C>& m) {
-    S* p = base + (int64_t)mu * 18 * stride + s;
+    S* p = base + (int64_t)mu * 18 * stride;
+    if constexpr (sizeof(S) == 4) {
+        const uint32_t o4 = (uint32_t)s * 16u, o2 = (uint32_t)s * 8u;
 #pragma unroll
-    for (int c = 0; c < 18; ++c) p[c * stride] = (S)m.v[c];
+        for (int k = 0; k < 4; ++k)
+            st_(p + (int64_t)k * 4 * stride, o4,
+                vf4{(float)m.v[4 * k], (float)m.v[4 * k + 1], (float)m.v[4 * k + 2], (float)m.v[4 * k + 3]});
+        st_(p + 16 * stride, o2, vf2{(float)m.v[16], (float)m.v[17]});
+    } else {
+        const uint32_t o = (uint32_t)s * 16u;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            st_(p + (int64_t)k * 2 * stride, o, vd2{(double)m.v[2 * k], (double)m.v[2 * k + 1]});
+    }
 }
 
 // Block -> tile mapping that keeps each XCD on a contiguous run of tiles.

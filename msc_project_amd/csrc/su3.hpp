@@ -35,69 +35,93 @@ LGF_DEV void set_unit(M3<R>& m) {
     m.v[0] = m.v[8] = m.v[16] = R(1);
 }
 
-// Products.  Suffix n = operand as is, d = operand daggered.
+// Products.  Suffix n = operand as is, d = operand daggered.  Every complex
+// multiply-add is written as four separate accumulations so that each
+// compiles to one FMA (x += a*b - c*d would leave a multiply and an add).
+// The _acc forms add the product into an existing matrix.
+#define LGF_CMAC(X, Y, AR, AI, BR, BI) \
+    X += (AR) * (BR);                  \
+    X -= (AI) * (BI);                  \
+    Y += (AR) * (BI);                  \
+    Y += (AI) * (BR);
+
+template <typename R>
+LGF_DEV void mul_nn_acc(M3<R>& o, const M3<R>& a, const M3<R>& b) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = o.v[2 * (3 * r + c)], y = o.v[2 * (3 * r + c) + 1];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                LGF_CMAC(x, y, a.v[2 * (3 * r + k)], a.v[2 * (3 * r + k) + 1], b.v[2 * (3 * k + c)],
+                         b.v[2 * (3 * k + c) + 1])
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+}
+
+// o += a * b^dag : (a b^dag)_{rc} = sum_k a_{rk} conj(b_{ck})
+template <typename R>
+LGF_DEV void mul_nd_acc(M3<R>& o, const M3<R>& a, const M3<R>& b) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = o.v[2 * (3 * r + c)], y = o.v[2 * (3 * r + c) + 1];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                LGF_CMAC(x, y, a.v[2 * (3 * r + k)], a.v[2 * (3 * r + k) + 1], b.v[2 * (3 * c + k)],
+                         -b.v[2 * (3 * c + k) + 1])
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+}
+
+// o += a^dag * b : sum_k conj(a_{kr}) b_{kc}
+template <typename R>
+LGF_DEV void mul_dn_acc(M3<R>& o, const M3<R>& a, const M3<R>& b) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = o.v[2 * (3 * r + c)], y = o.v[2 * (3 * r + c) + 1];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                LGF_CMAC(x, y, a.v[2 * (3 * k + r)], -a.v[2 * (3 * k + r) + 1], b.v[2 * (3 * k + c)],
+                         b.v[2 * (3 * k + c) + 1])
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+}
+
 template <typename R>
 LGF_DEV M3<R> mul_nn(const M3<R>& a, const M3<R>& b) {
     M3<R> o;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            R x = R(0), y = R(0);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                R ar = a.v[2 * (3 * r + k)], ai = a.v[2 * (3 * r + k) + 1];
-                R br = b.v[2 * (3 * k + c)], bi = b.v[2 * (3 * k + c) + 1];
-                x += ar * br - ai * bi;
-                y += ar * bi + ai * br;
-            }
-            o.v[2 * (3 * r + c)] = x;
-            o.v[2 * (3 * r + c) + 1] = y;
-        }
+    for (int i = 0; i < 18; ++i) o.v[i] = R(0);
+    mul_nn_acc(o, a, b);
     return o;
 }
 
-// a * b^dag : (a b^dag)_{rc} = sum_k a_{rk} conj(b_{ck})
 template <typename R>
 LGF_DEV M3<R> mul_nd(const M3<R>& a, const M3<R>& b) {
     M3<R> o;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            R x = R(0), y = R(0);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                R ar = a.v[2 * (3 * r + k)], ai = a.v[2 * (3 * r + k) + 1];
-                R br = b.v[2 * (3 * c + k)], bi = -b.v[2 * (3 * c + k) + 1];
-                x += ar * br - ai * bi;
-                y += ar * bi + ai * br;
-            }
-            o.v[2 * (3 * r + c)] = x;
-            o.v[2 * (3 * r + c) + 1] = y;
-        }
+    for (int i = 0; i < 18; ++i) o.v[i] = R(0);
+    mul_nd_acc(o, a, b);
     return o;
 }
 
-// a^dag * b : sum_k conj(a_{kr}) b_{kc}
 template <typename R>
 LGF_DEV M3<R> mul_dn(const M3<R>& a, const M3<R>& b) {
     M3<R> o;
 #pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            R x = R(0), y = R(0);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                R ar = a.v[2 * (3 * k + r)], ai = -a.v[2 * (3 * k + r) + 1];
-                R br = b.v[2 * (3 * k + c)], bi = b.v[2 * (3 * k + c) + 1];
-                x += ar * br - ai * bi;
-                y += ar * bi + ai * br;
-            }
-            o.v[2 * (3 * r + c)] = x;
-            o.v[2 * (3 * r + c) + 1] = y;
-        }
+    for (int i = 0; i < 18; ++i) o.v[i] = R(0);
+    mul_dn_acc(o, a, b);
     return o;
 }
 
@@ -118,7 +142,10 @@ LGF_DEV R retr_mul(const M3<R>& a, const M3<R>& b) {
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k)
-            s += a.v[2 * (3 * r + k)] * b.v[2 * (3 * k + r)] - a.v[2 * (3 * r + k) + 1] * b.v[2 * (3 * k + r) + 1];
+        {
+            s += a.v[2 * (3 * r + k)] * b.v[2 * (3 * k + r)];
+            s -= a.v[2 * (3 * r + k) + 1] * b.v[2 * (3 * k + r) + 1];
+        }
     return s;
 }
 

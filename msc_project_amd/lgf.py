@@ -59,6 +59,7 @@ SIGNATURES = {
     "lgf_ctx_set_stage_variant": ([_vp, ctypes.c_int, ctypes.c_int], ctypes.c_int),
     "lgf_ctx_get_stage_variant": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
     "lgf_num_stage_variants": ([ctypes.c_int, _ip], ctypes.c_int),
+    "lgf_autotune": ([_vp, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp], ctypes.c_int),
     "lgf_field_footprint": ([_ip, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "lgf_field_create": ([_vp, _ip, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "lgf_field_destroy": ([_vp], ctypes.c_int),
@@ -158,6 +159,15 @@ class Context:
 
     def set_stage_variant(self, precision, variant):
         self.check(lib().lgf_ctx_set_stage_variant(self.h, precision, variant))
+
+    def autotune(self, shape, precision, nsteps=10):
+        """Times every stage-kernel variant; selects and returns (best, ms_per_step list)."""
+        n = num_stage_variants(precision)
+        ms = np.zeros(n)
+        best = ctypes.c_int()
+        self.check(lib().lgf_autotune(self.h, _shape(shape), precision, nsteps, ctypes.byref(best), _p(ms)),
+                   "autotune")
+        return best.value, ms.tolist()
 
     def stage_variant(self, precision):
         v = ctypes.c_int()

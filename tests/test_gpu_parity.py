@@ -205,3 +205,24 @@ def test_fp32_tracks_fp64(ctx, config1):
     print("fp32: max rel d(t^2E) %.2e  max |dQ| %.2e" % (dE[1:].max(), np.abs(Q32 - Q64).max()))
     assert np.abs(Q32 - Q64).max() < 1e-4
     assert dE[1:].max() < 1e-5
+
+
+def test_stage_variants_agree_and_autotune(ctx):
+    """every stage-kernel variant computes the same links; the autotuner picks one of them"""
+    shape = (8, 4, 6, 4)
+    U = O.generate(shape, 6.0, seed=31, start="hot", sweeps=1)
+    for prec in (lgf.FP64, lgf.FP32):
+        outs = []
+        for v in range(lgf.num_stage_variants(prec)):
+            ctx.set_stage_variant(prec, v)
+            f = lgf.Field(ctx, shape, prec)
+            f.upload(U)
+            f.flow(0.02, 3)
+            outs.append(f.download())
+            f.close()
+        for o in outs[1:]:
+            assert np.array_equal(o, outs[0])
+        best, ms = ctx.autotune(shape, prec, 2)
+        assert 0 <= best < len(ms) and all(m > 0 for m in ms)
+        assert ctx.stage_variant(prec) == best
+        ctx.set_stage_variant(prec, 0)
