@@ -112,7 +112,7 @@ def run_path(lgf, ctx, U, prec, steps, warmup, ev):
     f.close()
     return {"wall_s": wall, "flow_ms_per_step": flow_ms, "measure_ms": meas_ms, "E": float(E[0]),
             "Q": float(Q[0]), "E_native": float(E_native[0]), "Q_native": float(Q_native[0]),
-            "unitarity_max": diag[1], "stage_variant": variant,
+            "unitarity_max": diag[1], "stage_variant": variant, "tile_order": ctx.tile_order(),
             "autotune_ms_per_step": tune_ms}
 
 
@@ -207,6 +207,7 @@ def main():
         "fp64_flow_only_lups": lu_step / (r64["flow_ms_per_step"] * 1e-3),
         "fp32_over_fp64_flow": r64["flow_ms_per_step"] / r32["flow_ms_per_step"],
         "stage_variant": {"fp64": r64["stage_variant"], "fp32": r32["stage_variant"]},
+        "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]},
         "measure_ms": {"fp32_links": r32["measure_ms"], "fp64_links": r64["measure_ms"]},
         "fp32_vs_fp64_final": {"dE_rel": abs(r32["E"] - r64["E"]) / abs(r64["E"]), "dQ": abs(r32["Q"] - r64["Q"])},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -87,9 +87,15 @@ int lgf_ctx_synchronize(lgf_ctx* ctx);
 int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
 int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
 int lgf_num_stage_variants(int precision, int* n);
-/* time every variant for nsteps RK3 steps on a seeded lattice of `shape`,
- * select the fastest for this context; ms_per_step (optional) gets one entry
- * per variant */
+/* traversal order of the lattice tiles in batches created afterwards:
+ * 0 = lexicographic, B > 1 = x0 blocked by B (better cache reuse of the
+ * x0-neighbours); results do not depend on it */
+int lgf_ctx_set_tile_order(lgf_ctx* ctx, int block);
+int lgf_ctx_get_tile_order(lgf_ctx* ctx, int* block);
+/* choose the tile order (fastest of 0, 2, 4, 8 with variant 0), then time
+ * every stage variant with it for nsteps RK3 steps on a seeded lattice of
+ * `shape`, and select the fastest for this context; ms_per_step (optional)
+ * gets one entry per variant */
 int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best, double* ms_per_step);
 
 /* ---- one lattice ------------------------------------------------------ */
@@ -105,6 +111,13 @@ int lgf_field_set_cold(lgf_field* f);
 /* seeded Haar-random SU(3) links (a "hot start"; same Philox4x32-10 counters
  * as the CPU oracle's, values equal to rounding) */
 int lgf_field_set_random(lgf_field* f, uint64_t seed);
+/* seeded configuration generator: nsweeps iterations of [one Cabibbo-Marinari
+ * heatbath sweep + or_per_hb overrelaxation sweeps] at coupling beta, with
+ * iteration indices first_sweep.. so that a chain can be continued; links
+ * are reunitarised after every iteration.  Same algorithm and Philox
+ * counters as the CPU oracle's generator (the reference's own generator,
+ * LGF/SU3_Heatbath/SU3_Heatbath.cpp:16-224, is not seedable). */
+int lgf_field_heatbath(lgf_field* f, double beta, uint64_t seed, int first_sweep, int nsweeps, int or_per_hb);
 /* copy links of src into dst (same shape; precision converted) */
 int lgf_field_copy(lgf_field* dst, const lgf_field* src);
 /* read/write a configuration file of the reference's format at `path`

@@ -369,6 +369,25 @@ extern "C" int lgf_measure(lgf_field* f, int flags, double* E, double* Q, double
     return rc;
 }
 
+// Seeded configuration generator (generator.hip): nsweeps iterations of one
+// heatbath sweep plus or_per_hb overrelaxation sweeps, iteration indices
+// first_sweep.. (so a chain can be continued), links reunitarised after each
+// iteration.  Start from lgf_field_set_cold / lgf_field_set_random.
+extern "C" int lgf_field_heatbath(lgf_field* f, double beta, uint64_t seed, int first_sweep, int nsweeps,
+                                  int or_per_hb) {
+    if (!f || nsweeps < 0 || or_per_hb < 0 || first_sweep < 0 || !(beta > 0)) return LGF_E_ARG;
+    lgf_ctx* ctx = f->ctx;
+    lgf_batch* b;
+    int rc = solo(f, &b);
+    if (rc) return rc;
+    rc = batch_sync_desc(b);
+    if (rc) return rc;
+    for (int it = 0; it < nsweeps; ++it)
+        HIPTRY(ctx, launch_heatbath_iteration(f->prec, b->d_desc, f->vol, b->ping, beta, seed,
+                                              (uint32_t)(first_sweep + it), or_per_hb, ctx->stream));
+    return LGF_OK;
+}
+
 extern "C" int lgf_diagnostics(lgf_field* f, double* plaq, double* umax, double* umean) {
     if (!f) return LGF_E_ARG;
     lgf_ctx* ctx = f->ctx;
@@ -390,6 +409,45 @@ extern "C" int lgf_diagnostics(lgf_field* f, double* plaq, double* umax, double*
 // ---------------------------------------------------------------------------
 // batches
 // ---------------------------------------------------------------------------
+// Processing order of a field's 64-site tiles.  Memory stays lexicographic
+// (x3 fastest); the order in which workgroups visit tiles decides how far
+// apart in time a link and its neighbours are touched.  In lexicographic
+// order the x0-neighbour of a site is a whole 3-d slab (N1*N2*N3 sites)
+// away, beyond an XCD's L2.  With block size B > 0 tiles are visited by the
+// key (x0 / B, x1, x0 % B, x2, x3): the x0-neighbour is then N2*N3 sites
+// away for all but one x0 in B, and the x1-neighbour B*N2*N3 sites away.
+// B = 0 keeps lexicographic order.  Reductions are indexed by the tile,
+// not by its place in this order, so results do not depend on B.
+static std::vector<int> batch_tile_order(const lgf_field* f, int B) {
+    std::vector<int> order(f->ntiles);
+    for (int t = 0; t < f->ntiles; ++t) order[t] = t;
+    if (B <= 1) return order;
+    const int64_t n1 = f->shape[1], n2 = f->shape[2], n3 = f->shape[3];
+    auto key = [&](int t) {
+        int64_t s = (int64_t)t * kTileSites;
+        const int64_t x3 = s % n3;
+        s /= n3;
+        const int64_t x2 = s % n2;
+        s /= n2;
+        const int64_t x1 = s % n1;
+        const int64_t x0 = s / n1;
+        return ((((x0 / B) * n1 + x1) * B + x0 % B) * n2 + x2) * n3 + x3;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
+    return order;
+}
+
+extern "C" int lgf_ctx_set_tile_order(lgf_ctx* ctx, int block) {
+    if (!ctx || block < 0 || block > 64) return LGF_E_ARG;
+    ctx->tile_block = block;
+    return LGF_OK;
+}
+
+extern "C" int lgf_ctx_get_tile_order(lgf_ctx* ctx, int* block) {
+    if (!ctx || !block) return LGF_E_ARG;
+    *block = ctx->tile_block;
+    return LGF_OK;
+}
 extern "C" int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfields, lgf_batch** out) {
     if (!ctx || !fields || nfields <= 0 || !out) return LGF_E_ARG;
     *out = nullptr;
@@ -413,14 +471,17 @@ extern "C" int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfie
         d.links[1] = f->links[1];
         d.acc = f->acc;
         d.qx = f->qx;
-        for (int k = 0; k < 4; ++k) d.n[k] = f->shape[k];
+        for (int k = 0; k < 4; ++k) {
+            d.n[k] = f->shape[k];
+            d.rcp[k] = (1.0 + 1.0 / 1099511627776.0) / (double)f->shape[k];
+        }
         d.vol = f->vol;
         d.ntiles = f->ntiles;
         d.stride = f->stride;
         d.eps = 0.0;
         d.tile0 = (int)tiles.size();
         d.flip = f->cur;
-        for (int t = 0; t < f->ntiles; ++t) tiles.push_back(make_int2(i, t));
+        for (int t : batch_tile_order(f, ctx->tile_block)) tiles.push_back(make_int2(i, t));
     }
     b->ntiles = (int)tiles.size();
     b->ping = 0;
@@ -592,6 +653,35 @@ extern "C" int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int
     int arg = 0;
     double best_ms = 1e300;
     const double eps = 0.01;
+    // 1. traversal order, with the default variant
+    {
+        static const int orders[4] = {0, 2, 4, 8};
+        int best_order = ctx->tile_block;
+        double best_order_ms = 1e300;
+        ctx->stage_variant[precision] = 0;
+        for (int oi = 0; oi < 4 && !rc; ++oi) {
+            ctx->tile_block = orders[oi];
+            lgf_batch* bo = nullptr;
+            rc = lgf_batch_create(ctx, &f, 1, &bo);
+            if (!rc) rc = lgf_batch_flow_steps(bo, &eps, 1, 0);
+            if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = LGF_E_HIP;
+            if (!rc) rc = lgf_batch_flow_steps(bo, &eps, nsteps, 0);
+            if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LGF_E_HIP;
+            float ms = 0.f;
+            if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+                rc = lgf_fail(ctx, LGF_E_HIP, "event timing");
+            if (bo) lgf_batch_destroy(bo);
+            if (!rc && ms < best_order_ms) {
+                best_order_ms = ms;
+                best_order = orders[oi];
+            }
+        }
+        ctx->tile_block = best_order;
+        if (b) lgf_batch_destroy(b);
+        b = nullptr;
+        if (!rc) rc = lgf_batch_create(ctx, &f, 1, &b);
+    }
+    // 2. stage-kernel variant, with that order
     for (int v = 0; v < nv && !rc; ++v) {
         ctx->stage_variant[precision] = v;
         rc = lgf_batch_flow_steps(b, &eps, 1, 0);

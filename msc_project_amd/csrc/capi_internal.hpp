@@ -15,6 +15,7 @@ struct lgf_ctx {
     hipStream_t stream = nullptr;
     std::string err;
     int stage_variant[2] = {0, 0};
+    int tile_block = 4;       // traversal order of new batches: see batch_tile_order
     void* staging = nullptr;  // device staging chunk for uploads/downloads
     void* pinned = nullptr;   // pinned host staging chunk
     int live_fields = 0;      // fields and user batches not yet destroyed

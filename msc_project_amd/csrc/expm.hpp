@@ -10,10 +10,9 @@
 // exp_ah_series: the same f0 + f1 Q + f2 Q^2 decomposition, but with the f_j
 //   summed from the exponential series using the Cayley-Hamilton recurrence
 //   Q^{n+1} = c0 d_n + (a_n + c1 d_n) Q + b_n Q^2 for Q^n = a_n + b_n Q + d_n Q^2.
-//   No transcendental functions; NTERMS = 13 gives a truncation error below
-//   |lambda|^13/13! (< 2e-10 for |lambda| <= 1, where lambda are the
-//   eigenvalues of Q).  Larger exponents are scaled and squared.  Used by
-//   the mixed-precision path.
+//   No transcendental functions; truncation below 1e-10 (see the bound at
+//   exp_ah_series).  Larger exponents are scaled and squared.  Used by the
+//   mixed-precision path.
 #pragma once
 #include "su3.hpp"
 
@@ -90,11 +89,15 @@ LGF_DEV M3<double> exp_ah_cayley(const M3<double>& X) {
 }
 
 // Power series through the Cayley-Hamilton recurrence, real type R.
-// Valid for c1 <= 0.75 (all |lambda| <= 1); larger exponents (rare: a flow
-// step moves a link by eps*|Z|) are scaled by 2^-k into that range and the
-// result squared k times, all in precision R.
-template <typename R, int NTERMS = 13>
+// Truncated after Q^(NTERMS-1): with c1 <= C1MAX every eigenvalue satisfies
+// |lambda| <= sqrt(4 c1 / 3) and the first omitted term is below
+// |lambda|^NTERMS / NTERMS!.  The defaults (10 terms, c1 <= 0.15, so
+// |lambda| <= 0.45) bound it by 1e-10, far below FP32 rounding.  Larger
+// exponents (rare: a flow step moves a link by eps*|Z|) are scaled by 2^-k
+// into that range and the result squared k times, all in precision R.
+template <typename R, int NTERMS = 10>
 LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
+    constexpr float C1MAX = 0.15f;
     M3<R> Q;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
@@ -104,9 +107,9 @@ LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
     M3<R> Q2 = mul_nn(Q, Q);
     R c1 = retr(Q2) * R(0.5);
     int k = 0;
-    if (c1 > R(0.75)) {  // scaling: c1 ~ |Q|^2 shrinks by 4 per halving
+    if (c1 > R(C1MAX)) {  // scaling: c1 ~ |Q|^2 shrinks by 4 per halving
         R scale = R(1);
-        while (c1 > R(0.75) && k < 30) {
+        while (c1 > R(C1MAX) && k < 30) {
             c1 *= R(0.25);
             scale *= R(0.5);
             ++k;

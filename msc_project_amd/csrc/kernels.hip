@@ -25,6 +25,7 @@
 #include "kernels.hpp"
 #include "lattice.hpp"
 #include "philox.hpp"
+#include "staple.hpp"
 #include "su3.hpp"
 
 namespace lgf {
@@ -50,31 +51,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8)
     S* __restrict__ Xa = (S*)d.acc;
     const int64_t st = d.stride;
     const Geo g = make_geo(d, s);
-    const int fmu = nb_off(g, mu, +1);
-
-    // A_mu(x) = sum_{nu != mu} U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag
-    //                        + (U_mu(x-nu) U_nu(x-nu+mu))^dag U_nu(x-nu)
-    M3<C> A;
-    set_zero(A);
-#pragma unroll 1
-    for (int k = 0; k < 3; ++k) {
-        const int nu = k + (k >= mu ? 1 : 0);
-        const int fnu = nb_off(g, nu, +1), bnu = nb_off(g, nu, -1);
-        {
-            M3<C> a = load_link<S, C>(U, st, nu, s);
-            M3<C> b = load_link<S, C>(U, st, mu, s + fnu);
-            M3<C> tt = mul_nn(a, b);
-            M3<C> c = load_link<S, C>(U, st, nu, s + fmu);
-            mul_nd_acc(A, c, tt);
-        }
-        {
-            M3<C> a = load_link<S, C>(U, st, mu, s + bnu);
-            M3<C> b = load_link<S, C>(U, st, nu, s + bnu + fmu);
-            M3<C> tt = mul_nn(a, b);
-            M3<C> c = load_link<S, C>(U, st, nu, s + bnu);
-            mul_dn_acc(A, tt, c);
-        }
-    }
+    const M3<C> A = staple<S, C>(U, st, g, mu);
     const M3<C> Umu = load_link<S, C>(U, st, mu, s);
     const M3<C> Z = ta_scaled(mul_nn(Umu, A), C(-0.5));
 

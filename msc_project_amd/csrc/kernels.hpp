@@ -26,6 +26,8 @@ hipError_t launch_from_soa(int prec, const void* src, double* dst, int64_t strid
                            hipStream_t stream);
 hipError_t launch_convert(int prec_in, const void* src, int prec_out, void* dst, int64_t stride, int vol,
                           hipStream_t stream);
+hipError_t launch_heatbath_iteration(int prec, const FieldDesc* fds, int vol, int ping, double beta, uint64_t seed,
+                                     uint32_t sweep, int or_per_hb, hipStream_t stream);
 hipError_t launch_hot(int prec, void* dst, int64_t stride, int vol, uint64_t seed, hipStream_t stream);
 hipError_t launch_cold(int prec, void* dst, int64_t stride, int vol, hipStream_t stream);
 

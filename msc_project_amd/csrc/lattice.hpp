@@ -34,6 +34,7 @@ struct FieldDesc {
     int ntiles;
     int64_t stride;
     double eps;      // flow step of this field
+    double rcp[4];   // (1 + 2^-40) / n[d], for division-free coordinate decomposition
     int tile0;       // first global tile (block) index of this field
     int flip;        // the current link buffer is links[ping ^ flip]
 };
@@ -47,6 +48,12 @@ struct Geo {
     int s;
 };
 
+// t / n for 0 <= t < 2^31 and 2 <= n < 2^16 with one double multiply: rcp
+// is (1 + 2^-40)/n rounded; the bias keeps exact quotients from rounding
+// down and is far too small (t/n * 2^-40 < 1/n) to lift t/n across the next
+// integer.
+LGF_DEV int fdiv(int t, double rcp) { return (int)((double)t * rcp); }
+
 LGF_DEV Geo make_geo(const FieldDesc& d, int s) {
     Geo g;
     g.s = s;
@@ -54,13 +61,15 @@ LGF_DEV Geo make_geo(const FieldDesc& d, int s) {
     g.n1 = d.n[1];
     g.n2 = d.n[2];
     g.n3 = d.n[3];
-    int t = s;
-    g.x3 = t % g.n3;
-    t /= g.n3;
-    g.x2 = t % g.n2;
-    t /= g.n2;
-    g.x1 = t % g.n1;
-    g.x0 = t / g.n1;
+    int t = s, q = fdiv(t, d.rcp[3]);
+    g.x3 = t - q * g.n3;
+    t = q;
+    q = fdiv(t, d.rcp[2]);
+    g.x2 = t - q * g.n2;
+    t = q;
+    q = fdiv(t, d.rcp[1]);
+    g.x1 = t - q * g.n1;
+    g.x0 = q;
     return g;
 }
 

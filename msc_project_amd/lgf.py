@@ -60,6 +60,8 @@ SIGNATURES = {
     "lgf_ctx_get_stage_variant": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
     "lgf_num_stage_variants": ([ctypes.c_int, _ip], ctypes.c_int),
     "lgf_autotune": ([_vp, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp], ctypes.c_int),
+    "lgf_ctx_set_tile_order": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lgf_ctx_get_tile_order": ([_vp, _ip], ctypes.c_int),
     "lgf_field_footprint": ([_ip, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "lgf_field_create": ([_vp, _ip, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "lgf_field_destroy": ([_vp], ctypes.c_int),
@@ -68,6 +70,8 @@ SIGNATURES = {
     "lgf_field_download": ([_vp, _dp], ctypes.c_int),
     "lgf_field_set_cold": ([_vp], ctypes.c_int),
     "lgf_field_set_random": ([_vp, ctypes.c_uint64], ctypes.c_int),
+    "lgf_field_heatbath": ([_vp, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int],
+                           ctypes.c_int),
     "lgf_field_copy": ([_vp, _vp], ctypes.c_int),
     "lgf_field_load_file": ([_vp, ctypes.c_char_p], ctypes.c_int),
     "lgf_field_save_file": ([_vp, ctypes.c_char_p], ctypes.c_int),
@@ -169,6 +173,14 @@ class Context:
                    "autotune")
         return best.value, ms.tolist()
 
+    def set_tile_order(self, block):
+        self.check(lib().lgf_ctx_set_tile_order(self.h, int(block)))
+
+    def tile_order(self):
+        v = ctypes.c_int()
+        self.check(lib().lgf_ctx_get_tile_order(self.h, ctypes.byref(v)))
+        return v.value
+
     def stage_variant(self, precision):
         v = ctypes.c_int()
         self.check(lib().lgf_ctx_get_stage_variant(self.h, precision, ctypes.byref(v)))
@@ -235,6 +247,10 @@ class Field:
 
     def set_random(self, seed):
         self.ctx.check(lib().lgf_field_set_random(self.h, int(seed)), "set_random")
+
+    def heatbath(self, beta, seed, sweeps, or_per_hb=4, first_sweep=0):
+        self.ctx.check(lib().lgf_field_heatbath(self.h, float(beta), int(seed), int(first_sweep), int(sweeps),
+                                                int(or_per_hb)), "heatbath")
 
     def copy_from(self, other):
         self.ctx.check(lib().lgf_field_copy(self.h, other.h), "copy")

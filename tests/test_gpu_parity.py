@@ -225,4 +225,40 @@ def test_stage_variants_agree_and_autotune(ctx):
         best, ms = ctx.autotune(shape, prec, 2)
         assert 0 <= best < len(ms) and all(m > 0 for m in ms)
         assert ctx.stage_variant(prec) == best
+        assert ctx.tile_order() in (0, 2, 4, 8)
         ctx.set_stage_variant(prec, 0)
+    # the traversal order changes neither links nor observables
+    res = []
+    for order in (0, 2, 4):
+        ctx.set_tile_order(order)
+        f = lgf.Field(ctx, shape, lgf.FP64)
+        f.upload(U)
+        f.flow(0.02, 3)
+        res.append((f.download(), f.measure()[:3]))
+        f.close()
+    ctx.set_tile_order(4)
+    for r in res[1:]:
+        assert np.array_equal(r[0], res[0][0]) and r[1] == res[0][1]
+
+
+@pytest.mark.slow
+def test_large24_golden_fp64(ctx):
+    """First 50 steps of the 24^4, beta 6.0 reference flow (eps 0.01, every 5).
+    Slow: the input is regenerated by the CPU oracle's generator (200 sweeps)."""
+    import hashlib
+    import os
+    if not os.environ.get("LGF_SLOW"):
+        pytest.skip("set LGF_SLOW=1: regenerates a 24^4 input on the CPU (minutes)")
+    d = golden("large24")
+    g = d["generator"]
+    shape = tuple(d["shape"])
+    U = O.generate(shape, d["beta"], seed=g["seed"], start=g["start"], sweeps=g["therm_sweeps"],
+                   or_per_hb=g["or_per_hb"])
+    assert hashlib.sha256(U.tobytes()).hexdigest() == d["input_sha256"][0]
+    c = d["configs"][0]
+    f = lgf.Field(ctx, shape, lgf.FP64)
+    f.upload(U)
+    t, E, Q, Qa = flow_measure(f, d["eps"], d["steps"], d["every"])
+    print("24^4: max rel dE %.2e, max dQ/scale %.2e" % (rel_err(E, c["E"]), q_err(Q, c["Q"], Qa)))
+    assert rel_err(E, c["E"]) < TOL_FP64
+    assert q_err(Q, c["Q"], Qa) < TOL_FP64
