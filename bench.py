@@ -1,23 +1,30 @@
 """Benchmark of the Wilson-flow path on one MI355X.
 
-Workload (the single-lattice configuration of the headline metric): one
-24^4 lattice at beta = 6.0, Luescher RK3 with eps = 0.01, E, t^2E and Q
-measured every 5 steps.  A "step" is one RK3 step of the whole lattice
-(4*24^4 link-updates) plus, every 5th step, one clover measurement.
+Main workload (the single-lattice configuration of the headline metric): one
+24^4 lattice at beta = 6.0, generated on the GPU from a seeded cold start by
+200 heatbath/overrelaxation iterations (1 HB + 4 OR each), then flowed with
+Luescher RK3, eps = 0.01, for 500 steps (t = 5), measuring E, t^2E and Q every
+5 steps.  A "step" is one RK3 step of the whole lattice (4*24^4 link-updates)
+with its share of the measurements.
 
-Reported `value`: link-updates/s of the FP32-arithmetic path with FP64
-reductions and FP64 measurements, whole job (flow + measurements), links
-resident in HBM.  Also reported: the same for the FP64 parity path, the
-flow-only rates, the roofline of the RK3 stage kernels (HIP events on the
-library's stream) and a CPU baseline (the CPU oracle on a bounded sample).
+`value`: link-updates/s of the whole job on the FP32-arithmetic path (FP32
+links, staples, exponentials and intermediate clovers; FP64 reductions,
+FP64 reunitarisation every step and an FP64 final measurement sweep), links
+resident in HBM.  Reported beside it: the FP64 parity path on the same input,
+flow-only rates, the accuracy of FP32 against FP64 at every measured flow
+time and at t0 (t^2 E = 0.3), the roofline of the RK3 stage kernels (HIP
+events on the library's stream), a batched run of 16 uneven lattices, a
+continuous-batching serving run with p50/p99 latency, and a CPU baseline.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps 500] [--warmup 10]
+    python bench.py --profile fp32|fp64|fp32native --steps 6   (fixed workload for rocprofv3)
 """
 import argparse
 import ctypes
 import json
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -26,8 +33,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 SHAPE = (24, 24, 24, 24)
+BETA = 6.0
 EPS = 0.01
 MEASURE_EVERY = 5
+THERM_SWEEPS = 200
+SEED = 24242424
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_LU = {0: 1440.0, 1: 720.0}  # (6 s_U + 4 s_X) * 18 bytes per link-update
 
@@ -41,7 +51,6 @@ class HipEvents:
         self.h.hipEventRecord.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         self.h.hipEventSynchronize.argtypes = [ctypes.c_void_p]
         self.h.hipEventElapsedTime.argtypes = [ctypes.POINTER(ctypes.c_float), ctypes.c_void_p, ctypes.c_void_p]
-        self.h.hipEventDestroy.argtypes = [ctypes.c_void_p]
 
     def event(self):
         e = ctypes.c_void_p()
@@ -58,90 +67,196 @@ class HipEvents:
         return ms.value
 
 
-def make_input(seed=1234):
-    """Seeded Haar-random links (no thermalisation): the flow's cost does
-    not depend on link values except through the exponential's branch mix."""
-    from oracle import oracle as O
-    return O.generate(SHAPE, 6.0, seed=seed, start="hot", sweeps=0)
+def log(msg):
+    print("# " + msg, file=sys.stderr, flush=True)
 
 
-def run_path(lgf, ctx, U, prec, steps, warmup, ev):
+def thermalised(lgf, ctx, shape, beta, seed, sweeps):
+    f = lgf.Field(ctx, shape, lgf.FP64)
+    f.set_cold()
+    f.heatbath(beta, seed, sweeps)
+    return f
+
+
+def run_job(lgf, ctx, src, prec, steps, warmup, ev):
+    """One flow job on a copy of `src`: returns timings and the measured series."""
     variant, tune_ms = ctx.autotune(SHAPE, prec, 6)
-    f = lgf.Field(ctx, SHAPE, prec)
-    f.upload(U)
-    b = lgf.Batch(ctx, [f])
-    # intermediate measurements: clover arithmetic in the path's precision with
-    # FP64 reductions; the final sweep below is always FP64
-    flags = lgf.MEASURE_DEFAULT
-    # warmup (also captures the graphs)
-    for s in range(warmup):
-        b.flow(EPS, 1, use_graph=True)
-    b.flow(EPS, 2, use_graph=True)
-    b.measure(flags)
-    ctx.synchronize()
+    flags = lgf.MEASURE_DEFAULT  # clover arithmetic in the path's precision, FP64 reductions
+    # warm-up on a scratch copy (captures graphs, touches every kernel)
+    w = lgf.Field(ctx, SHAPE, prec)
+    w.copy_from(src)
+    wb = lgf.Batch(ctx, [w])
+    wb.flow(EPS, max(2, warmup), use_graph=True)
+    wb.measure(flags)
     stream = ctx.stream()
-    # flow-only timing of the stage kernels with HIP events
     e0, e1 = ev.event(), ev.event()
-    nflow = max(2, steps - steps % 2)
     ev.record(e0, stream)
-    b.flow(EPS, nflow, use_graph=True)
+    wb.flow(EPS, 20, use_graph=True)
     ev.record(e1, stream)
-    flow_ms = ev.elapsed_ms(e0, e1) / nflow
-    # one measurement
-    m0, m1 = ev.event(), ev.event()
-    ev.record(m0, stream)
-    b.measure(flags)
-    ev.record(m1, stream)
-    meas_ms = ev.elapsed_ms(m0, m1)
-    # whole job: steps with a measurement every MEASURE_EVERY steps
+    flow_ms = ev.elapsed_ms(e0, e1) / 20
+    ev.record(e0, stream)
+    wb.measure(flags)
+    ev.record(e1, stream)
+    meas_ms = ev.elapsed_ms(e0, e1)
+    wb.close()
+    w.close()
+    # the timed job
+    f = lgf.Field(ctx, SHAPE, prec)
+    f.copy_from(src)
+    b = lgf.Batch(ctx, [f])
     ctx.synchronize()
     t0 = time.perf_counter()
+    E, Q = [], []
+    e, q, _ = b.measure(flags)
+    E.append(e[0]), Q.append(q[0])
     done = 0
     while done < steps:
         n = min(MEASURE_EVERY, steps - done)
         b.flow(EPS, n, use_graph=True)
         done += n
-        if done % MEASURE_EVERY == 0:
-            b.measure(flags)
+        if done % MEASURE_EVERY == 0 or done == steps:
+            e, q, _ = b.measure(lgf.MEASURE_FP64 if done == steps else flags)  # final sweep in FP64
+            E.append(e[0]), Q.append(q[0])
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    E, Q, Qa = b.measure(lgf.MEASURE_FP64)
-    E_native, Q_native, _ = b.measure(flags)
     diag = f.diagnostics()
     b.close()
     f.close()
-    return {"wall_s": wall, "flow_ms_per_step": flow_ms, "measure_ms": meas_ms, "E": float(E[0]),
-            "Q": float(Q[0]), "E_native": float(E_native[0]), "Q_native": float(Q_native[0]),
-            "unitarity_max": diag[1], "stage_variant": variant, "tile_order": ctx.tile_order(),
-            "autotune_ms_per_step": tune_ms}
+    t = np.array([0.0] + [EPS * k for k in range(MEASURE_EVERY, steps + 1, MEASURE_EVERY)])[: len(E)]
+    return {"wall_s": wall, "flow_ms_per_step": flow_ms, "measure_ms": meas_ms, "stage_variant": variant,
+            "tile_order": ctx.tile_order(), "autotune_ms_per_step": tune_ms, "t": t, "E": np.array(E),
+            "Q": np.array(Q), "unitarity_max": diag[1]}
 
 
-def cpu_baseline(U, budget_s=20.0):
-    """The CPU oracle (a scalar-per-thread C port of the reference path) on
-    whole RK3 steps of the same 24^4 lattice, within a time budget."""
+def t0_of(t, E):
+    y = t ** 2 * E
+    for i in range(1, len(t)):
+        if y[i - 1] < 0.3 <= y[i]:
+            return t[i - 1] + (0.3 - y[i - 1]) * (t[i] - t[i - 1]) / (y[i] - y[i - 1])
+    return None
+
+
+def accuracy(r64, r32):
+    t = r64["t"]
+    y64, y32 = t ** 2 * r64["E"], t ** 2 * r32["E"]
+    rel = np.abs(y32 - y64)[1:] / np.abs(y64[1:])
+    dq = np.abs(r32["Q"] - r64["Q"])
+    out = {"max_abs_dQ": float(dq.max()), "max_rel_dt2E": float(rel.max()), "measurements": int(len(t)),
+           "final_abs_dQ": float(dq[-1]), "final_rel_dt2E": float(rel[-1]), "final_Q_fp64": float(r64["Q"][-1])}
+    t0 = t0_of(t, r64["E"])
+    if t0 is not None:
+        i = int(np.searchsorted(t, t0))
+        w = (t0 - t[i - 1]) / (t[i] - t[i - 1])
+
+        def interp(a):
+            return (1 - w) * a[i - 1] + w * a[i]
+
+        out.update({"t0": float(t0), "abs_dQ_at_t0": float(interp(dq)),
+                    "rel_dt2E_at_t0": float(abs(interp(y32) - interp(y64)) / interp(y64))})
+    return out
+
+
+def batched(lgf, ctx):
+    """16 uneven lattices (6 x 12^4 at beta 5.8, 6 x 16^4 at 6.0, 4 x 20^3x40 at 6.1),
+    each from 200 seeded heatbath/OR iterations, flowed together in FP32 to t = 3
+    (300 steps of eps 0.01) with graph-captured steps, measuring every 10 steps."""
+    specs = [((12, 12, 12, 12), 5.8)] * 6 + [((16, 16, 16, 16), 6.0)] * 6 + [((20, 20, 20, 40), 6.1)] * 4
+    fields = []
+    for i, (shape, beta) in enumerate(specs):
+        g = thermalised(lgf, ctx, shape, beta, 1000 + i, THERM_SWEEPS)
+        f = lgf.Field(ctx, shape, lgf.FP32)
+        f.copy_from(g)
+        g.close()
+        fields.append(f)
+    b = lgf.Batch(ctx, fields)
+    b.flow(EPS, 2, use_graph=True)  # capture
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for k in range(30):
+        b.flow(EPS, 10, use_graph=True)
+        b.measure(lgf.MEASURE_DEFAULT)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    lu = 4.0 * sum(np.prod(s) for s, _ in specs) * 300
+    b.close()
+    for f in fields:
+        f.close()
+    return {"configs": len(specs), "link_updates": lu, "wall_s": wall, "link_updates_per_s": lu / wall,
+            "steps": 300, "measure_every": 10, "precision": "fp32"}
+
+
+def serving(lgf, nreq, throughput, seed=4242):
+    """Poisson stream of flow requests (shape uniform in {16^4, 24^4, 32^4,
+    32^3x64}, beta in {5.96, 6.05, 6.13, 6.21}, eps 0.01, t uniform in [1, 6],
+    measure every 10 steps, seeded random start) at a rate set for ~80%
+    utilisation of the measured FP32 throughput."""
+    rng = np.random.default_rng(seed)
+    shapes = [(16, 16, 16, 16), (24, 24, 24, 24), (32, 32, 32, 32), (32, 32, 32, 64)]
+    betas = [5.96, 6.05, 6.13, 6.21]
+    reqs = []
+    for i in range(nreq):
+        shape = shapes[rng.integers(4)]
+        t = rng.uniform(1.0, 6.0)
+        reqs.append((shape, betas[rng.integers(4)], int(round(t / EPS)), 100000 + i))
+    mean_work = np.mean([4.0 * np.prod(s) * n for s, _, n, _ in reqs])
+    rate = 0.8 * throughput / mean_work
+    gaps = rng.exponential(1.0 / rate, size=nreq)
+    srv = lgf.Server(0)
+    ids = [None] * nreq
+
+    def feeder():
+        t_next = time.perf_counter()
+        for i, (shape, beta, n, sd) in enumerate(reqs):
+            t_next += gaps[i]
+            dt = t_next - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            ids[i] = srv.submit(shape, EPS, n, 10, precision=lgf.FP32, seed=sd, beta=beta,
+                                measure_flags=lgf.MEASURE_DEFAULT)
+
+    t0 = time.perf_counter()
+    th = threading.Thread(target=feeder)
+    th.start()
+    th.join()
+    res = [srv.wait(i, timeout=600) for i in ids]
+    wall = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["finish_s"] - r["submit_s"] for r in res])
+    ok = all(r["status"] == 0 for r in res)
+    lu = sum(4.0 * np.prod(s) * n for s, _, n, _ in reqs)
+    return {"requests": nreq, "arrival_rate_per_s": rate, "all_ok": ok, "p50_latency_s": float(np.percentile(lat, 50)),
+            "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
+            "throughput_link_updates_per_s": lu / wall, "wall_s": wall, "busy_s": st["busy_s"],
+            "batch_steps": st["batch_steps"]}
+
+
+def cpu_baseline(budget_s=20.0):
+    """The CPU oracle (a C port of the reference path, OpenMP over sites) on
+    whole RK3 steps of a 24^4 lattice (seeded Haar-random links; the flow's
+    CPU cost does not depend on the values), within a time budget."""
     from oracle import oracle as O
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
-    V = U.copy()
+    V = O.generate(SHAPE, BETA, seed=1234, start="hot", sweeps=0)
     t0 = time.perf_counter()
     n = 0
     while True:
         O.rk3_step(SHAPE, V, EPS)
         n += 1
-        if time.perf_counter() - t0 > budget_s / 2 or n >= 4:
+        if time.perf_counter() - t0 > budget_s / 2:
             break
     dt = time.perf_counter() - t0
-    lu = 4.0 * np.prod(SHAPE) * n
-    return {"value": lu / dt, "unit": "link-updates/s", "cores": threads, "kind": "port",
-            "sample": "%d RK3 step(s) of the 24^4 lattice with the CPU oracle (oracle/lgf_oracle.c, "
-                      "OpenMP over sites), no measurements, %.1f s" % (n, dt)}
+    return {"value": 4.0 * np.prod(SHAPE) * n / dt, "unit": "link-updates/s", "cores": threads, "kind": "port",
+            "sample": "%d RK3 steps of a 24^4 lattice with the CPU oracle (oracle/lgf_oracle.c, %d OpenMP "
+                      "threads), no measurements, %.1f s" % (n, threads, dt)}
 
 
 def profile_run(which, steps):
-    """Fixed workload for rocprofv3 counter runs: `steps` RK3 steps of the
-    24^4 lattice in one precision (direct launches, default stage variant),
-    one clover measurement, and one plain copy of a link array (k_convert of
-    72*stride elements: a kernel of known traffic to calibrate the counters)."""
+    """Fixed workload for rocprofv3 counter runs: `steps` RK3 steps of a
+    seeded random 24^4 lattice in one precision (direct launches), one clover
+    measurement, and one plain copy of a link array (k_convert, a kernel of
+    known traffic that calibrates the counters)."""
     from msc_project_amd import lgf
     ctx = lgf.Context(0)
     prec = lgf.FP64 if which == "fp64" else lgf.FP32
@@ -159,12 +274,25 @@ def profile_run(which, steps):
                       "stage_algorithmic_bytes_per_step": BYTES_PER_LU[prec] * 4 * int(np.prod(SHAPE))}))
 
 
+def traffic_from_profiles():
+    """HBM bytes per launch of the FP32 stage kernel (mean of the three
+    stages) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
+    p = os.path.join(ROOT, "profiles", "rk3_stage_fp32_24x24x24x24_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            d = json.load(f)
+        return d.get("hbm_bytes_per_launch")
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--steps", type=int, default=500, help="RK3 steps of the timed job")
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=40, help="0 disables the serving run")
     ap.add_argument("--profile", choices=["fp32", "fp64", "fp32native"], default=None,
                     help="run the fixed counter-profiling workload instead of the benchmark")
     args = ap.parse_args()
@@ -177,14 +305,24 @@ def main():
         sys.exit(2)
     from msc_project_amd import lgf
     ev = HipEvents()
-    U = make_input()
     ctx = lgf.Context(0)
+    tg = time.perf_counter()
+    src = thermalised(lgf, ctx, SHAPE, BETA, SEED, THERM_SWEEPS)
+    ctx.synchronize()
+    plaq = src.diagnostics()[0]
+    log("input: 24^4 beta 6.0, %d HB/OR iterations in %.1f s, <P> = %.5f" % (THERM_SWEEPS, time.perf_counter() - tg,
+                                                                             plaq))
     res = {}
     for prec in (lgf.FP64, lgf.FP32):
-        res[prec] = run_path(lgf, ctx, U, prec, args.steps, args.warmup, ev)
-        print("# %s: %s" % ("fp64" if prec == 0 else "fp32", json.dumps(res[prec])), file=sys.stderr, flush=True)
+        res[prec] = run_job(lgf, ctx, src, prec, args.steps, args.warmup, ev)
+        r = res[prec]
+        log("%s: job %.3f s, flow %.4f ms/step, measure %.3f ms, variant %d, tile order %d" % (
+            "fp64" if prec == 0 else "fp32", r["wall_s"], r["flow_ms_per_step"], r["measure_ms"], r["stage_variant"],
+            r["tile_order"]))
     lu_step = 4.0 * np.prod(SHAPE)
     r32, r64 = res[lgf.FP32], res[lgf.FP64]
+    acc = accuracy(r64, r32)
+    log("accuracy: %s" % json.dumps(acc))
     value = lu_step * args.steps / r32["wall_s"]
     achieved = BYTES_PER_LU[1] * lu_step / (r32["flow_ms_per_step"] * 1e-3) / 1e9
     line = {
@@ -199,23 +337,33 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32",
-        "data": "synthetic: seeded Haar-random 24^4 SU(3) links",
-        "config": {"workload": "single 24^4 lattice, beta 6.0, RK3 eps 0.01, E/t^2E/Q every 5 steps",
+        "data": "synthetic: 24^4 beta 6.0 from a seeded cold start and 200 GPU heatbath/OR iterations "
+                "(<P> = %.5f)" % plaq,
+        "config": {"workload": "single 24^4 lattice, beta 6.0, RK3 eps 0.01, 500 steps (t=5), E/t^2E/Q every 5 steps",
                    "lattice": list(SHAPE), "eps": EPS, "measure_every": MEASURE_EVERY},
-        "fp32_flow_only_lups": lu_step / (r32["flow_ms_per_step"] * 1e-3),
         "fp64_lups": lu_step * args.steps / r64["wall_s"],
+        "fp32_over_fp64_job": r64["wall_s"] / r32["wall_s"],
+        "fp32_flow_only_lups": lu_step / (r32["flow_ms_per_step"] * 1e-3),
         "fp64_flow_only_lups": lu_step / (r64["flow_ms_per_step"] * 1e-3),
         "fp32_over_fp64_flow": r64["flow_ms_per_step"] / r32["flow_ms_per_step"],
-        "stage_variant": {"fp64": r64["stage_variant"], "fp32": r32["stage_variant"]},
-        "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]},
-        "measure_ms": {"fp32_links": r32["measure_ms"], "fp64_links": r64["measure_ms"]},
-        "fp32_vs_fp64_final": {"dE_rel": abs(r32["E"] - r64["E"]) / abs(r64["E"]), "dQ": abs(r32["Q"] - r64["Q"])},
+        "measure_ms": {"fp32": r32["measure_ms"], "fp64": r64["measure_ms"]},
+        "fp32_vs_fp64": acc,
+        "unitarity_max": {"fp32": r32["unitarity_max"], "fp64": r64["unitarity_max"]},
+        "tuning": {"stage_variant": {"fp64": r64["stage_variant"], "fp32": r32["stage_variant"]},
+                   "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "k_rk3_stage<float> x3 per step (720 B per link-update algorithmic)"},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "kernel": "k_rk3_stage<float> (3 launches per RK3 step, 720 B per link-update algorithmic)"},
     }
+    if not args.no_batch:
+        line["batched_16_uneven"] = batched(lgf, ctx)
+        log("batched: %s" % json.dumps(line["batched_16_uneven"]))
+    if args.serve_requests > 0:
+        line["serving"] = serving(lgf, args.serve_requests, value)
+        line["p99_latency_s"] = line["serving"]["p99_latency_s"]
+        log("serving: %s" % json.dumps(line["serving"]))
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(U)
+        line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)
 
 
