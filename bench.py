@@ -260,12 +260,14 @@ def profile_run(which, steps):
     from msc_project_amd import lgf
     ctx = lgf.Context(0)
     prec = lgf.FP64 if which == "fp64" else lgf.FP32
+    src = lgf.Field(ctx, SHAPE, prec)  # copy source, cold by the time it is copied
+    src.set_random(99)
     f = lgf.Field(ctx, SHAPE, prec)
     f.set_random(1234)
-    f.flow(EPS, steps)
+    f.flow(EPS, steps)  # > 1 GB of traffic: evicts src from the 256 MB last-level cache
     f.measure(lgf.MEASURE_FP64 if which != "fp32native" else lgf.MEASURE_DEFAULT)
     g = lgf.Field(ctx, SHAPE, prec)
-    g.copy_from(f)
+    g.copy_from(src)
     ctx.synchronize()
     stride = (np.prod(SHAPE) + 63) // 64 * 64
     nbytes = 72 * stride * (8 if prec == lgf.FP64 else 4)
