@@ -34,7 +34,9 @@ LGF_DEV M3<double> exp_ah_cayley(const M3<double>& X) {
     double c0 = retr_mul(Q, Q2) / 3.0;
     double c1 = retr(Q2) / 2.0;
     double c1_sqrt = sqrt(c1);
-    double ratio = fabs(c0) / (2.0 * pow((c1 / 3.0), 1.5));
+    // clamped as the reference's _DEBUG build does (LGF/Math/SU3_mat.cpp:230-238);
+    // equal to the release build wherever that one is finite
+    double ratio = fmin(fabs(c0) / (2.0 * pow((c1 / 3.0), 1.5)), 1.0);
     double theta3 = acos(ratio) / 3.0;
     double u = sqrt(1.0 / 3.0) * c1_sqrt * cos(theta3);
     double u2 = u * u;

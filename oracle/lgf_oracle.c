@@ -164,7 +164,11 @@ void orc_exp_ah(const double *X, double *out) {
     double c0 = m_retr(Q3) / 3.0;
     double c1 = m_retr(Q2) / 2.0;
     double c1_sqrt = sqrt(c1);
-    double ratio = fabs(c0) / (2.0 * pow((c1 / 3.0), 1.5));
+    /* the reference clamps the acos argument only in _DEBUG builds
+       (LGF/Math/SU3_mat.cpp:230-238); a release build turns a rounding
+       excess over 1 into NaN.  Clamped here: identical wherever the
+       reference is finite. */
+    double ratio = fmin(fabs(c0) / (2.0 * pow((c1 / 3.0), 1.5)), 1.0);
     double theta3 = acos(ratio) / 3.0;
     double u = sqrt(1.0 / 3.0) * c1_sqrt * cos(theta3);
     double u2 = u * u;

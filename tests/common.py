@@ -7,7 +7,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 GOLDEN = os.path.join(HERE, "golden")
 
-# FP64 parity bar (relative), the north-star tolerance for E and Q per measurement
+# FP64 parity bar (relative) for E and Q per measurement against the reference
 TOL_FP64 = 1e-12
 
 
