@@ -92,6 +92,12 @@ int lgf_num_stage_variants(int precision, int* n);
  * x0-neighbours); results do not depend on it */
 int lgf_ctx_set_tile_order(lgf_ctx* ctx, int block);
 int lgf_ctx_get_tile_order(lgf_ctx* ctx, int* block);
+/* device scratch a measurement may hold for its line products (2- and
+ * 3-link products of every link, 576 B per site for FP32 clover arithmetic,
+ * 1152 B for FP64): the fields of a batch are measured in consecutive groups
+ * that fit `bytes`, a single larger field on its own.  Default 1 GiB.  The
+ * scratch is kept by the context until it is destroyed. */
+int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes);
 /* choose the tile order (fastest of 0, 2, 4, 8 with variant 0), then time
  * every stage variant with it for nsteps RK3 steps on a seeded lattice of
  * `shape`, and select the fastest for this context; ms_per_step (optional)

@@ -8,6 +8,7 @@
 // call launches the kernels of kernels.hip or returns an error.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -36,7 +37,8 @@ int check_shape(const int* shape, int64_t* vol) {
         if (shape[i] <= 0 || (shape[i] & 1)) return LGF_E_SHAPE;  // even extents (LGF/Lattice/Lattice.cpp:42-49)
         v *= shape[i];
     }
-    if (v * 4 * 18 >= (int64_t(1) << 62) || v >= (int64_t(1) << 31) - 64) return LGF_E_SHAPE;
+    // kernels address a vector plane with 32-bit byte offsets of 16 B per site
+    if (v > (int64_t(1) << 28)) return LGF_E_SHAPE;
     *vol = v;
     return LGF_OK;
 }
@@ -92,6 +94,7 @@ extern "C" int lgf_ctx_destroy(lgf_ctx* ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     if (ctx->staging) (void)hipFree(ctx->staging);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->lines) (void)hipFree(ctx->lines);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return LGF_OK;
@@ -102,6 +105,12 @@ extern "C" const char* lgf_ctx_last_error(const lgf_ctx* ctx) { return ctx ? ctx
 extern "C" int lgf_ctx_stream(lgf_ctx* ctx, void** s) {
     if (!ctx || !s) return LGF_E_ARG;
     *s = (void*)ctx->stream;
+    return LGF_OK;
+}
+
+extern "C" int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes) {
+    if (!ctx) return LGF_E_ARG;
+    ctx->lines_cap = bytes;
     return LGF_OK;
 }
 
@@ -611,7 +620,33 @@ extern "C" int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, 
     if (rc) return rc;
     const int compute = (flags & LGF_MEASURE_FP64) ? LGF_FP64 : b->prec;
     const int n = (int)b->fields.size();
-    HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, b->ntiles, b->ping, b->d_partials, ctx->stream));
+    // consecutive fields are measured together while their line products
+    // fit ctx->lines_cap (a single larger field is measured on its own)
+    const size_t per_tile = line_bytes_per_site(compute) * kTileSites;
+    std::vector<std::pair<int, int>> chunks;  // (first tile, tiles)
+    size_t need = 0;
+    for (int i = 0; i < n;) {
+        int t0 = b->h_desc[i].tile0, nt = 0;
+        while (i < n && (nt == 0 || size_t(nt + b->h_desc[i].ntiles) * per_tile <= ctx->lines_cap))
+            nt += b->h_desc[i++].ntiles;
+        chunks.emplace_back(t0, nt);
+        need = std::max(need, size_t(nt) * per_tile);
+    }
+    if (need > ctx->lines_bytes) {
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->lines) (void)hipFree(ctx->lines);
+        ctx->lines = nullptr;
+        ctx->lines_bytes = 0;
+        if (hipMalloc(&ctx->lines, need) != hipSuccess) {
+            (void)hipGetLastError();
+            ctx->lines = nullptr;
+            return lgf_fail(ctx, LGF_E_NOMEM, "no device memory for the clover line products");
+        }
+        ctx->lines_bytes = need;
+    }
+    for (const auto& c : chunks)
+        HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, c.first, c.second, b->ping, ctx->lines,
+                                  b->d_partials, ctx->stream));
     HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials, 3, b->d_out, ctx->stream));
     HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));

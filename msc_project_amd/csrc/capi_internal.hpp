@@ -18,6 +18,9 @@ struct lgf_ctx {
     int tile_block = 4;       // traversal order of new batches: see batch_tile_order
     void* staging = nullptr;  // device staging chunk for uploads/downloads
     void* pinned = nullptr;   // pinned host staging chunk
+    void* lines = nullptr;    // line products of the clover measurement (grown on demand)
+    size_t lines_bytes = 0;
+    size_t lines_cap = size_t(1) << 30;  // fields are measured in chunks of about this many line bytes
     int live_fields = 0;      // fields and user batches not yet destroyed
     int live_batches = 0;
 };

@@ -97,29 +97,45 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8)
 // ---------------------------------------------------------------------------
 // clovers, E, Q, q(x)
 // ---------------------------------------------------------------------------
-// Clover leaves.
+// Line products.  L^d_l(y) = U_d(y) U_d(y+d) ... U_d(y+(l-1)d), the product
+// of l links along a straight line.  k_lines stores L_2 and L_3 of every
+// site and direction of the fields being measured (precision C, the vector
+// plane layout of the links, line index lbase + site); L_1 is the link.
 //
-// A leaf of extent (la x lb) in the quadrant spanned by the signed
-// directions a, b starts at x, takes la steps along a, lb along b, la back
-// along -a and lb back along -b.  A step along +d multiplies by U_d(here),
-// a step along -d by U_d(here - d)^dag.  The four quadrants of the (mu,nu)
-// plane are those of the reference's leaves, (a,b) in
-// {(+nu,+mu), (+mu,-nu), (-mu,+nu), (-nu,-mu)}; e.g. the first 1x1 leaf is
-// U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag U_mu(x)^dag (LGF/Field/SU3_field.cpp:
-// 550-560), and the 2x2, 1x2 and 1x3 clovers (LGF/Field/SU3_field.cpp:
-// 598-792) are the same four leaves with longer legs.
-//
-// Every link of a leaf is an "a-link" (along a's axis) between a-positions i
-// and i+1 at b-position j, or a "b-link".  Their site offsets are computed
-// once per quadrant; the walk itself is unrolled at compile time, and the
-// dagger of a step is folded into which components are loaded (a uniform,
-// scalar choice) and the sign of the imaginary parts.
-struct Quadrant {
-    int off_a[4], off_b[4];  // site offset of i steps along a (j along b)
-    int lnk_a[3], lnk_b[3];  // offset of the site holding the i-th a-link (j-th b-link)
-    int dir_a, dir_b;        // axes of a and b
-    bool neg_a, neg_b;       // a (b) points against its axis
+// With them every clover leaf of extent m along mu and n along nu is three
+// matrix products.  The reference's four quadrant leaves of the (mu,nu)
+// plane (LGF/Field/SU3_field.cpp:533-584 for 1x1, 598-792 for 2x2, 1x2 and
+// 1x3), each a closed loop from x, are:
+//   quadrant (+nu,+mu): W = P(x) R(x)^dag
+//   quadrant (-nu,-mu): W = R(c)^dag P(c),            c = x - n nu - m mu
+//     P(c) = L^nu_n(c) L^mu_m(c + n nu),  R(c) = L^mu_m(c) L^nu_n(c + m mu)
+//   quadrant (+mu,-nu): W = L^mu_m(x) H^dag L^nu_n(z), z = x - n nu,
+//     H = L^mu_m(z) L^nu_n(z + m mu)
+//   quadrant (-mu,+nu): W^dag = L^nu_n(x) H'^dag L^mu_m(z'), z' = x - m mu,
+//     H' = L^nu_n(z') L^mu_m(z' + n nu)
+// (e.g. the first 1x1 leaf is U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag U_mu(x)^dag,
+// LGF/Field/SU3_field.cpp:550-560).  The field strength depends on a clover
+// sum only through C - C^dag, to which W and -W^dag contribute alike, so the
+// last quadrant enters as -W^dag; the two "side" quadrants then share one
+// code path with mu and nu exchanged, and the two "diagonal" quadrants share
+// one up to the last product.  Leaf shapes come in both orientations with
+// equal weights, so a shape loop over (l1, l2) covers both.
+template <typename S, typename C>
+struct Lines {
+    const S* U;   // current links: lines of length 1
+    int64_t st;   // link stride
+    const C* L2;  // lines of length 2
+    const C* L3;  // lines of length 3
+    int64_t G;    // line stride
+    int lbase;    // line index of site 0 of this field
 };
+
+template <int LEN, typename S, typename C>
+LGF_DEV M3<C> line(const Lines<S, C>& L, int dir, int site) {
+    if constexpr (LEN == 1) return load_link<S, C>(L.U, L.st, dir, site);
+    else if constexpr (LEN == 2) return load_link<C, C>(L.L2, L.G, dir, L.lbase + site);
+    else return load_link<C, C>(L.L3, L.G, dir, L.lbase + site);
+}
 
 LGF_DEV int wrap_offset(const Geo& g, int dir, int steps) {
     const int xd = geo_x(g, dir), nd = geo_n(g, dir);
@@ -128,88 +144,46 @@ LGF_DEV int wrap_offset(const Geo& g, int dir, int steps) {
     return (v - xd) * geo_str(g, dir);
 }
 
-LGF_DEV Quadrant make_quadrant(const Geo& g, int dir_a, bool neg_a, int dir_b, bool neg_b) {
-    Quadrant q;
-    q.dir_a = dir_a;
-    q.dir_b = dir_b;
-    q.neg_a = neg_a;
-    q.neg_b = neg_b;
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        q.off_a[i] = wrap_offset(g, dir_a, neg_a ? -i : i);
-        q.off_b[i] = wrap_offset(g, dir_b, neg_b ? -i : i);
-    }
-#pragma unroll
-    for (int i = 0; i < 3; ++i) {
-        q.lnk_a[i] = neg_a ? q.off_a[i + 1] : q.off_a[i];
-        q.lnk_b[i] = neg_b ? q.off_b[i + 1] : q.off_b[i];
-    }
-    return q;
-}
-
-// U_dir at site s, or its conjugate transpose when dag (uniform): a
-// branch-free select per element, so the matrix stays in registers
-template <typename S, typename C>
-LGF_DEV M3<C> load_link_maybe_dag(const S* __restrict__ U, int64_t st, int dir, int s, bool dag) {
-    const M3<C> m = load_link<S, C>(U, st, dir, s);
-    M3<C> t;
-    const C sgn = dag ? C(-1) : C(1);
-#pragma unroll
-    for (int r = 0; r < 3; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int a = 2 * (3 * r + c), b = 2 * (3 * c + r);
-            t.v[a] = (r == c) ? m.v[a] : (dag ? m.v[b] : m.v[a]);
-            t.v[a + 1] = sgn * ((r == c) ? m.v[a + 1] : (dag ? m.v[b + 1] : m.v[a + 1]));
-        }
-    return t;
-}
-
-// Step k of the leaf walk (compile-time k after unrolling): axis, site and
-// whether the link is traversed backwards (daggered).
-template <int LA, int LB>
-LGF_DEV void leaf_link(int k, int s, const Quadrant& q, int& dir, int& site, bool& dag) {
-    if (k < LA) {  // out along a, row j = 0
-        dir = q.dir_a, site = s + q.lnk_a[k] + q.off_b[0], dag = q.neg_a;
-    } else if (k < LA + LB) {  // out along b, column i = LA
-        dir = q.dir_b, site = s + q.off_a[LA] + q.lnk_b[k - LA], dag = q.neg_b;
-    } else if (k < 2 * LA + LB) {  // back along -a, row j = LB
-        dir = q.dir_a, site = s + q.lnk_a[2 * LA + LB - 1 - k] + q.off_b[LB], dag = !q.neg_a;
-    } else {  // back along -b, column i = 0
-        dir = q.dir_b, site = s + q.off_a[0] + q.lnk_b[2 * LA + 2 * LB - 1 - k], dag = !q.neg_b;
-    }
-}
-
-// Product of the 2(LA+LB) links of a leaf.  The next link is always in
-// flight while the current product is formed; the empty asm with a memory
-// clobber keeps the compiler from hoisting loads further ahead, which would
-// hold whole leaves in registers.
-template <typename S, typename C, int LA, int LB>
-LGF_DEV M3<C> leaf(const S* __restrict__ U, int64_t st, int s, const Quadrant& q) {
-    constexpr int N = 2 * (LA + LB);
-    int dir, site;
-    bool dag;
-    leaf_link<LA, LB>(0, s, q, dir, site, dag);
-    M3<C> P = load_link_maybe_dag<S, C>(U, st, dir, site, dag);
-    leaf_link<LA, LB>(1, s, q, dir, site, dag);
-    M3<C> Lnext = load_link_maybe_dag<S, C>(U, st, dir, site, dag);
-#pragma unroll
-    for (int k = 1; k < N; ++k) {
-        const M3<C> L = Lnext;
-        if (k + 1 < N) {
-            leaf_link<LA, LB>(k + 1, s, q, dir, site, dag);
-            Lnext = load_link_maybe_dag<S, C>(U, st, dir, site, dag);
-        }
-        asm volatile("" ::: "memory");
-        P = mul_nn(P, L);
-    }
-    return P;
-}
+// site offsets of x + k mu (m[3 + k]) and x + k nu (n[3 + k]), |k| <= 3;
+// offsets along different axes add, offsets along one axis do not (wrap)
+struct PlaneOff {
+    int m[7], n[7];
+};
 
 template <typename C>
-LGF_DEV void axpy(M3<C>& acc, C w, const M3<C>& P) {
+LGF_DEV void scale(M3<C>& a, C w) {
 #pragma unroll
-    for (int i = 0; i < 18; ++i) acc.v[i] += w * P.v[i];
+    for (int i = 0; i < 18; ++i) a.v[i] *= w;
+}
+
+// diagonal quadrants: it = 0 is (+nu,+mu), it = 1 is (-nu,-mu)
+template <int M, int N, typename S, typename C>
+LGF_DEV void diag_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
+    asm volatile("" ::: "memory");  // keep the next leaf's loads from being hoisted into this one
+    const int mb = o.m[3 - M], nb = o.n[3 - N], mf = o.m[3 + M], nf = o.n[3 + N];
+    const int sc = it ? s + nb + mb : s;
+    const int scn = it ? s + mb : s + nf;  // c + n nu
+    const int scm = it ? s + nb : s + mf;  // c + m mu
+    M3<C> P = mul_nn(line<N>(L, nu, sc), line<M>(L, mu, scn));
+    const M3<C> R = mul_nn(line<M>(L, mu, sc), line<N>(L, nu, scm));
+    if (w != C(1)) scale(P, w);
+    if (it == 0) mul_nd_acc(acc, P, R);
+    else mul_dn_acc(acc, R, P);
+}
+
+// side quadrants: it = 0 is (+mu,-nu) with (alpha, beta) = (mu, nu); it = 1
+// is (-mu,+nu) with (alpha, beta) = (nu, mu), entering as -W^dag
+template <int L1, int L2, typename S, typename C>
+LGF_DEV void side_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
+    asm volatile("" ::: "memory");
+    const int a = it ? nu : mu, b = it ? mu : nu;
+    const int m2 = o.m[3 - L2], n2 = o.n[3 - L2], m1 = o.m[3 + L1], n1 = o.n[3 + L1];
+    const int zb = it ? m2 : n2;  // z = x - l2 beta
+    const int za = it ? n1 : m1;  // + l1 alpha
+    const M3<C> H = mul_nn(line<L1>(L, a, s + zb), line<L2>(L, b, s + zb + za));
+    M3<C> T = mul_nd(line<L1>(L, a, s), H);
+    scale(T, it ? -w : w);
+    mul_nn_acc(acc, T, line<L2>(L, b, s + zb));
 }
 
 // F = 1/2 (-i) [ (C - C^dag) - 1/3 Tr(C - C^dag) ] (LGF/Field/SU3_field.cpp:575-579)
@@ -225,35 +199,52 @@ LGF_DEV M3<C> project_F(const M3<C>& Cl) {
     return F;
 }
 
-// improved F of plane (mu,nu) and the plain 1x1 F, both projected
+// Plane (mu,nu): returns Re Tr(F11 F11) of the plain 1x1 F (energy density)
+// and the improved F = 19/9 F11 + 1/36 F22 - 32/45 F12 + 1/15 F13
+// (LGF/Field/SU3_field.cpp:588-597), where each F is the projection of its
+// clover average: 1/4 of four square leaves, 1/8 of eight rectangle leaves.
+// The projection is linear, so it is applied once to the weighted sum.
 template <typename S, typename C>
-LGF_DEV void plane_F(const S* __restrict__ U, int64_t st, const Geo& g, int mu, int nu, M3<C>& Fimp,
-                     M3<C>& F11) {
-    M3<C> Ci, C11;
-    set_zero(Ci);
-    set_zero(C11);
-    const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
-#pragma unroll 1
-    for (int qd = 0; qd < 4; ++qd) {
-        // (a,b) = (+nu,+mu), (+mu,-nu), (-mu,+nu), (-nu,-mu)
-        const int dir_a = (qd == 0 || qd == 3) ? nu : mu;
-        const int dir_b = (qd == 0 || qd == 3) ? mu : nu;
-        const bool neg_a = qd >= 2, neg_b = (qd & 1) != 0;
-        const Quadrant q = make_quadrant(g, dir_a, neg_a, dir_b, neg_b);
-        // (la, lb) = extents along (a, b); both orientations of each
-        // rectangle enter with the same weight
-        add_to(C11, leaf<S, C, 1, 1>(U, st, g.s, q));
-        axpy(Ci, C(0.25) * k2, leaf<S, C, 2, 2>(U, st, g.s, q));
-        axpy(Ci, C(0.125) * k3, leaf<S, C, 1, 2>(U, st, g.s, q));
-        axpy(Ci, C(0.125) * k3, leaf<S, C, 2, 1>(U, st, g.s, q));
-        axpy(Ci, C(0.125) * k4, leaf<S, C, 1, 3>(U, st, g.s, q));
-        axpy(Ci, C(0.125) * k4, leaf<S, C, 3, 1>(U, st, g.s, q));
-    }
-    axpy(Ci, C(0.25) * k1, C11);
+LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fimp) {
+    PlaneOff o;
 #pragma unroll
-    for (int i = 0; i < 18; ++i) C11.v[i] *= C(0.25);
-    Fimp = project_F(Ci);
-    F11 = project_F(C11);
+    for (int k = -3; k <= 3; ++k) {
+        o.m[3 + k] = wrap_offset(g, mu, k);
+        o.n[3 + k] = wrap_offset(g, nu, k);
+    }
+    const int s = g.s;
+    const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
+    M3<C> acc;
+    set_zero(acc);
+#pragma unroll 1
+    for (int it = 0; it < 2; ++it) diag_leaf<1, 1>(acc, C(1), L, s, o, mu, nu, it);
+#pragma unroll 1
+    for (int it = 0; it < 2; ++it) side_leaf<1, 1>(acc, C(1), L, s, o, mu, nu, it);
+    scale(acc, C(0.25));
+    C e;
+    {
+        const M3<C> F11 = project_F(acc);
+        e = retr_mul(F11, F11);
+    }
+    scale(acc, k1);
+#pragma unroll 1
+    for (int it = 0; it < 2; ++it) {
+        diag_leaf<2, 2>(acc, C(0.25) * k2, L, s, o, mu, nu, it);
+        diag_leaf<1, 2>(acc, C(0.125) * k3, L, s, o, mu, nu, it);
+        diag_leaf<2, 1>(acc, C(0.125) * k3, L, s, o, mu, nu, it);
+        diag_leaf<1, 3>(acc, C(0.125) * k4, L, s, o, mu, nu, it);
+        diag_leaf<3, 1>(acc, C(0.125) * k4, L, s, o, mu, nu, it);
+    }
+#pragma unroll 1
+    for (int it = 0; it < 2; ++it) {
+        side_leaf<2, 2>(acc, C(0.25) * k2, L, s, o, mu, nu, it);
+        side_leaf<1, 2>(acc, C(0.125) * k3, L, s, o, mu, nu, it);
+        side_leaf<2, 1>(acc, C(0.125) * k3, L, s, o, mu, nu, it);
+        side_leaf<1, 3>(acc, C(0.125) * k4, L, s, o, mu, nu, it);
+        side_leaf<3, 1>(acc, C(0.125) * k4, L, s, o, mu, nu, it);
+    }
+    Fimp = project_F(acc);
+    return e;
 }
 
 // Re Tr(A B) for Hermitian A, B given as 9 reals each:
@@ -276,16 +267,37 @@ LGF_DEV void herm_pack(const M3<C>& F, double* h) {
     h[8] = (double)F.v[11];
 }
 
+// L_2 and L_3 of one (site, direction) of a chunk of the batch's tiles
+// (tiles = the chunk's part of the tile list, t0 = its first tile).
+template <typename S, typename C>
+__global__ __launch_bounds__(256) void k_lines(const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles,
+                                               int t0, int ping, C* __restrict__ lines, int64_t G) {
+    const int tix = xcd_remap(blockIdx.x, gridDim.x);
+    const int2 tm = tiles[tix];
+    const FieldDesc& d = fds[tm.x];
+    const int s = tm.y * kTileSites + threadIdx.x;
+    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    if (s >= d.vol) return;
+    const S* __restrict__ U = (const S*)d.links[ping ^ d.flip];
+    const Geo g = make_geo(d, s);
+    const M3<C> L2 = mul_nn(load_link<S, C>(U, d.stride, mu, s), load_link<S, C>(U, d.stride, mu, s + nb_off(g, mu, +1)));
+    const int li = (d.tile0 - t0) * kTileSites + s;
+    store_link<C, C>(lines, G, mu, li, L2);
+    store_link<C, C>(lines + 4 * 18 * G, G, mu, li,
+                     mul_nn(L2, load_link<S, C>(U, d.stride, mu, s + wrap_offset(g, mu, 2))));
+}
+
 // One thread per (site, plane).  The six planes are taken in the order of
 // the reference's charge, (3,2),(0,1) | (3,1),(2,0) | (3,0),(1,2)
 // (LGF/Observables/TopologicalCharge.cpp:12-16); they visit each plane
 // once, which the energy density sums over (EnergyDensity.cpp:13-20).
 // Per site: e(x) = sum_planes Re Tr(F11 F11), q(x) = sum_pairs Re Tr(Fa Fb)
 // with the improved F; tiles write (sum e, sum q, sum |q|) partials and,
-// when requested, q(x).
+// when requested, q(x).  Reads the lines of the same chunk from k_lines.
 template <typename S, typename C>
 __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clover(
-    const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int ping, double* __restrict__ partials) {
+    const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int t0, int ping, const C* __restrict__ lines,
+    int64_t G, double* __restrict__ partials) {
     __shared__ double sF[6][9][kTileSites];
     __shared__ double sE[6][kTileSites];
     const int tix = xcd_remap(blockIdx.x, gridDim.x);
@@ -297,14 +309,19 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
     double e = 0.0;
     double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (live) {
-        const S* __restrict__ U = (const S*)d.links[ping ^ d.flip];
+        Lines<S, C> L;
+        L.U = (const S*)d.links[ping ^ d.flip];
+        L.st = d.stride;
+        L.L2 = lines;
+        L.L3 = lines + 4 * 18 * G;
+        L.G = G;
+        L.lbase = (d.tile0 - t0) * kTileSites;
         const Geo g = make_geo(d, s);
         // plane p: (3,2),(0,1),(3,1),(2,0),(3,0),(1,2)
         const int mu = (p & 1) ? (p == 1 ? 0 : (p == 3 ? 2 : 1)) : 3;
         const int nu = (p & 1) ? (p == 1 ? 1 : (p == 3 ? 0 : 2)) : (p == 0 ? 2 : (p == 2 ? 1 : 0));
-        M3<C> Fi, F11;
-        plane_F<S, C>(U, d.stride, g, mu, nu, Fi, F11);
-        e = (double)retr_mul(F11, F11);
+        M3<C> Fi;
+        e = (double)plane_F<S, C>(L, g, mu, nu, Fi);
         herm_pack(Fi, h);
     }
 #pragma unroll
@@ -541,15 +558,26 @@ hipError_t launch_rk3_stage(int prec, int stage, const FieldDesc* fds, const int
     return launch_rk3_stage_variant(prec, 0, stage, fds, tiles, ntiles, ping, stream);
 }
 
-hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
-                         double* partials, hipStream_t stream) {
-    dim3 grid(ntiles), block(kTileSites, 6);
-    if (prec == kPrecFP64)
-        hipLaunchKernelGGL((k_clover<double, double>), grid, block, 0, stream, fds, tiles, ping, partials);
-    else if (compute == kPrecFP64)
-        hipLaunchKernelGGL((k_clover<float, double>), grid, block, 0, stream, fds, tiles, ping, partials);
-    else
-        hipLaunchKernelGGL((k_clover<float, float>), grid, block, 0, stream, fds, tiles, ping, partials);
+size_t line_bytes_per_site(int compute) { return size_t(2 * 4 * 18) * (compute == kPrecFP64 ? 8 : 4); }
+
+hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2* tiles, int t0, int ntiles, int ping,
+                         void* lines, double* partials, hipStream_t stream) {
+    const int2* ct = tiles + t0;
+    const int64_t G = (int64_t)ntiles * kTileSites;
+    dim3 grid(ntiles), lblock(kTileSites, 4), block(kTileSites, 6);
+    if (prec == kPrecFP64) {
+        hipLaunchKernelGGL((k_lines<double, double>), grid, lblock, 0, stream, fds, ct, t0, ping, (double*)lines, G);
+        hipLaunchKernelGGL((k_clover<double, double>), grid, block, 0, stream, fds, ct, t0, ping,
+                           (const double*)lines, G, partials);
+    } else if (compute == kPrecFP64) {
+        hipLaunchKernelGGL((k_lines<float, double>), grid, lblock, 0, stream, fds, ct, t0, ping, (double*)lines, G);
+        hipLaunchKernelGGL((k_clover<float, double>), grid, block, 0, stream, fds, ct, t0, ping,
+                           (const double*)lines, G, partials);
+    } else {
+        hipLaunchKernelGGL((k_lines<float, float>), grid, lblock, 0, stream, fds, ct, t0, ping, (float*)lines, G);
+        hipLaunchKernelGGL((k_clover<float, float>), grid, block, 0, stream, fds, ct, t0, ping, (const float*)lines,
+                           G, partials);
+    }
     return hipGetLastError();
 }
 

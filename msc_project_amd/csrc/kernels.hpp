@@ -15,8 +15,13 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
                                     int ntiles, int ping, hipStream_t stream);
 hipError_t launch_rk3_stage(int prec, int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
                             hipStream_t stream);
-hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
-                         double* partials, hipStream_t stream);
+// bytes of line products (k_lines) per site for clover arithmetic in `compute`
+size_t line_bytes_per_site(int compute);
+// clover observables of the tiles [t0, t0 + ntiles) of a batch's tile list;
+// those tiles must cover whole fields, and `lines` must hold
+// ntiles * 64 * line_bytes_per_site(compute) bytes
+hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2* tiles, int t0, int ntiles, int ping,
+                         void* lines, double* partials, hipStream_t stream);
 hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int ncomp, double* out,
                          hipStream_t stream);
 hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int ntiles, int ping, double* partials,

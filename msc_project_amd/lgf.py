@@ -62,6 +62,7 @@ SIGNATURES = {
     "lgf_autotune": ([_vp, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp], ctypes.c_int),
     "lgf_ctx_set_tile_order": ([_vp, ctypes.c_int], ctypes.c_int),
     "lgf_ctx_get_tile_order": ([_vp, _ip], ctypes.c_int),
+    "lgf_ctx_set_measure_scratch": ([_vp, ctypes.c_size_t], ctypes.c_int),
     "lgf_field_footprint": ([_ip, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "lgf_field_create": ([_vp, _ip, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "lgf_field_destroy": ([_vp], ctypes.c_int),
@@ -175,6 +176,9 @@ class Context:
 
     def set_tile_order(self, block):
         self.check(lib().lgf_ctx_set_tile_order(self.h, int(block)))
+
+    def set_measure_scratch(self, nbytes):
+        self.check(lib().lgf_ctx_set_measure_scratch(self.h, int(nbytes)))
 
     def tile_order(self):
         v = ctypes.c_int()

@@ -64,7 +64,7 @@ def test_cold_start_is_fixed(ctx):
         assert np.array_equal(f.download(), cold_links(shape))
 
 
-@pytest.mark.parametrize("shape", [(4, 4, 4, 4), (6, 4, 8, 4), (4, 8, 4, 6)])
+@pytest.mark.parametrize("shape", [(4, 4, 4, 4), (6, 4, 8, 4), (4, 8, 4, 6), (2, 4, 6, 8), (8, 6, 2, 4)])
 def test_measure_matches_oracle(ctx, shape):
     U = O.generate(shape, 6.0, seed=sum(shape), start="hot", sweeps=3)
     f = lgf.Field(ctx, shape, lgf.FP64)
@@ -144,6 +144,27 @@ def test_batch_same_alone_and_in_mix(ctx):
         for i, f in enumerate(fields):
             assert np.array_equal(f.download(), alone[i][0])
             assert (E[i], Q[i], Qa[i]) == alone[i][1]
+
+
+def test_measure_in_groups_matches_one_launch(ctx):
+    """a small measurement scratch splits a batch into groups of fields;
+    every field's observables stay bitwise the same"""
+    shapes = [(4, 4, 4, 4), (6, 4, 4, 8), (8, 4, 6, 4), (4, 4, 4, 4)]
+    inputs = [O.generate(s, 6.0, seed=300 + i, start="hot", sweeps=1) for i, s in enumerate(shapes)]
+    for prec, flags in ((lgf.FP64, lgf.MEASURE_DEFAULT), (lgf.FP32, lgf.MEASURE_DEFAULT), (lgf.FP32, lgf.MEASURE_FP64)):
+        fields = [lgf.Field(ctx, s, prec) for s in shapes]
+        for f, U in zip(fields, inputs):
+            f.upload(U)
+        b = lgf.Batch(ctx, fields)
+        whole = b.measure(flags)
+        ctx.set_measure_scratch(1)  # every field on its own
+        alone = b.measure(flags)
+        ctx.set_measure_scratch(1 << 30)
+        for x, y in zip(whole, alone):
+            assert np.array_equal(x, y)
+        b.close()
+        for f in fields:
+            f.close()
 
 
 def test_graph_replay_matches_direct(ctx):
