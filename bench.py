@@ -39,7 +39,10 @@ MEASURE_EVERY = 5
 THERM_SWEEPS = 200
 SEED = 24242424
 HBM_PEAK_GBS = 8000.0
-BYTES_PER_LU = {0: 1440.0, 1: 720.0}  # (6 s_U + 4 s_X) * 18 bytes per link-update
+# algorithmic HBM bytes per link-update: 6 link transfers of 18 reals and 4
+# accumulator transfers of 9 reals (the accumulator is stored as its 9
+# independent reals), (6*18 + 4*9) * sizeof(real)
+BYTES_PER_LU = {0: 1152.0, 1: 576.0}
 
 
 class HipEvents:
@@ -355,7 +358,7 @@ def main():
                    "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "k_rk3_stage<float> (3 launches per RK3 step, 720 B per link-update algorithmic)"},
+                     "kernel": "k_rk3_stage<float> (3 launches per RK3 step, 576 B per link-update algorithmic)"},
     }
     if not args.no_batch:
         line["batched_16_uneven"] = batched(lgf, ctx)

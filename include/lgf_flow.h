@@ -75,6 +75,16 @@ enum lgf_measure_flags {
     LGF_MEASURE_FP64 = 1     /* clover arithmetic in FP64 also for FP32 fields */
 };
 
+/* observables of one clover sweep (lgf_measure_observables) */
+enum lgf_observable {
+    LGF_OBS_E = 0,            /* EnergyDensity: (1/V) sum_x sum_{mu<nu} ReTr(F F), 1x1 clover F */
+    LGF_OBS_Q = 1,            /* TopologicalCharge: 1/(4 pi^2) sum_x qraw(x), improved F */
+    LGF_OBS_QABS = 2,         /* 1/(4 pi^2) sum_x |qraw(x)|, a scale for judging Q differences */
+    LGF_OBS_E_IMPROVED = 3,   /* ImprovedEnergyDensity: as E with the improved F */
+    LGF_OBS_Q_UNIMPROVED = 4, /* UnImprovedTopologicalCharge: as Q with the 1x1 clover F */
+    LGF_NUM_OBS = 5
+};
+
 /* ---- context ---------------------------------------------------------- */
 int lgf_ctx_create(int device, lgf_ctx** out);
 int lgf_ctx_destroy(lgf_ctx* ctx);
@@ -139,6 +149,12 @@ int lgf_flow_steps(lgf_field* f, double eps, int nsteps);
  * qraw (optional, volume doubles at the global index) =
  *   ReTr(F32 F01) + ReTr(F31 F20) + ReTr(F30 F12) per site. */
 int lgf_measure(lgf_field* f, int flags, double* E, double* Q, double* Qabs, double* qraw);
+/* all lgf_observable values of one sweep: obs[LGF_NUM_OBS]; covers also
+ * ImprovedEnergyDensity::calculate (LGF/Observables/ImprovedEnergyDensity.cpp:9-32)
+ * and UnImprovedTopologicalCharge::calculate
+ * (LGF/Observables/UnImprovedTopologicalCharge.cpp:8-29), the observables of
+ * MeasureFlowedGaugeConfigurations (LGF/LGF.cpp:228-255) */
+int lgf_measure_observables(lgf_field* f, int flags, double* obs, double* qraw);
 int lgf_diagnostics(lgf_field* f, double* avg_plaquette, double* unitarity_max, double* unitarity_mean);
 
 /* ---- many lattices (any shapes) in one set of launches ----------------- */
@@ -148,6 +164,8 @@ int lgf_batch_destroy(lgf_batch* b);
  * captured hipGraph (two steps per graph). */
 int lgf_batch_flow_steps(lgf_batch* b, const double* eps, int nsteps, int use_graph);
 int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, double* Qabs);
+/* obs[LGF_NUM_OBS * field + lgf_observable] */
+int lgf_batch_measure_observables(lgf_batch* b, int flags, double* obs);
 
 /* ---- continuous-batching flow server ----------------------------------- */
 typedef struct {

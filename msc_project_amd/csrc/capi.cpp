@@ -156,7 +156,7 @@ extern "C" int lgf_field_footprint(const int shape[4], int precision, size_t* by
     int rc = check_shape(shape, &vol);
     if (rc) return rc;
     if (!bytes || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
-    *bytes = size_t(3) * 72 * size_t(round_stride(vol)) * elem_size(precision);
+    *bytes = size_t(2 * 72 + 36) * size_t(round_stride(vol)) * elem_size(precision);
     return LGF_OK;
 }
 
@@ -175,11 +175,13 @@ extern "C" int lgf_field_create(lgf_ctx* ctx, const int shape[4], int precision,
     f->vol = (int)vol;
     f->stride = round_stride(vol);
     f->ntiles = (int)((vol + kTileSites - 1) / kTileSites);
+    // two link buffers of 72 reals per site, and the accumulator of 36
+    // (9 per link: traceless anti-Hermitian, see lattice.hpp)
     const size_t bytes = size_t(72) * size_t(f->stride) * elem_size(precision);
     HIPTRY(ctx, hipSetDevice(ctx->device));
     void* p[3] = {nullptr, nullptr, nullptr};
     for (int i = 0; i < 3; ++i) {
-        if (hipMalloc(&p[i], bytes) != hipSuccess) {
+        if (hipMalloc(&p[i], i < 2 ? bytes : bytes / 2) != hipSuccess) {
             for (int j = 0; j < i; ++j) (void)hipFree(p[j]);
             delete f;
             (void)hipGetLastError();
@@ -228,7 +230,7 @@ extern "C" int lgf_field_set_cold(lgf_field* f) {
     if (!f) return LGF_E_ARG;
     lgf_ctx* ctx = f->ctx;
     HIPTRY(ctx, launch_cold(f->prec, f->links[f->cur], f->stride, f->vol, ctx->stream));
-    HIPTRY(ctx, launch_cold(f->prec, f->acc, f->stride, f->vol, ctx->stream));
+    HIPTRY(ctx, hipMemsetAsync(f->acc, 0, size_t(36) * size_t(f->stride) * elem_size(f->prec), ctx->stream));
     return LGF_OK;
 }
 
@@ -362,14 +364,14 @@ extern "C" int lgf_flow_steps(lgf_field* f, double eps, int nsteps) {
     return lgf_batch_flow_steps(b, &eps, nsteps, 0);
 }
 
-extern "C" int lgf_measure(lgf_field* f, int flags, double* E, double* Q, double* Qabs, double* qraw) {
-    if (!f) return LGF_E_ARG;
+extern "C" int lgf_measure_observables(lgf_field* f, int flags, double* obs, double* qraw) {
+    if (!f || !obs) return LGF_E_ARG;
     lgf_ctx* ctx = f->ctx;
     lgf_batch* b;
     int rc = solo(f, &b);
     if (rc) return rc;
     if (qraw && !f->qx) HIPTRY(ctx, hipMalloc(&f->qx, size_t(f->vol) * sizeof(double)));
-    rc = lgf_batch_measure(b, flags, E, Q, Qabs);
+    rc = lgf_batch_measure_observables(b, flags, obs);
     if (rc && rc != LGF_E_NAN) return rc;
     if (qraw) {
         HIPTRY(ctx, hipMemcpyAsync(qraw, f->qx, size_t(f->vol) * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
@@ -407,7 +409,7 @@ extern "C" int lgf_diagnostics(lgf_field* f, double* plaq, double* umax, double*
     if (rc) return rc;
     HIPTRY(ctx, launch_diag(f->prec, b->d_desc, b->d_tiles, b->ntiles, b->ping, b->d_partials, 1, b->d_out,
                             ctx->stream));
-    HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, 4 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     if (plaq) *plaq = b->h_out[0] / (3.0 * 6.0 * (double)f->vol);
     if (umean) *umean = b->h_out[1] / (4.0 * (double)f->vol);
@@ -496,9 +498,10 @@ extern "C" int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfie
     b->ping = 0;
     hipError_t e = hipMalloc(&b->d_desc, sizeof(FieldDesc) * nfields);
     if (e == hipSuccess) e = hipMalloc(&b->d_tiles, sizeof(int2) * tiles.size());
-    if (e == hipSuccess) e = hipMalloc(&b->d_partials, sizeof(double) * 3 * tiles.size());
-    if (e == hipSuccess) e = hipMalloc(&b->d_out, sizeof(double) * 4 * nfields);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_out, sizeof(double) * 4 * nfields, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMalloc(&b->d_partials, sizeof(double) * kCloverParts * tiles.size());
+    if (e == hipSuccess) e = hipMalloc(&b->d_out, sizeof(double) * kOutStride * nfields);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&b->h_out, sizeof(double) * kOutStride * nfields, hipHostMallocDefault);
     if (e == hipSuccess)
         e = hipMemcpy(b->d_tiles, tiles.data(), sizeof(int2) * tiles.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -613,8 +616,31 @@ extern "C" int lgf_batch_flow_steps(lgf_batch* b, const double* eps, int nsteps,
     return LGF_OK;
 }
 
+extern "C" int lgf_measure(lgf_field* f, int flags, double* E, double* Q, double* Qabs, double* qraw) {
+    double obs[LGF_NUM_OBS];
+    int rc = lgf_measure_observables(f, flags, obs, qraw);
+    if (rc && rc != LGF_E_NAN) return rc;
+    if (E) *E = obs[LGF_OBS_E];
+    if (Q) *Q = obs[LGF_OBS_Q];
+    if (Qabs) *Qabs = obs[LGF_OBS_QABS];
+    return rc;
+}
+
 extern "C" int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, double* Qabs) {
     if (!b) return LGF_E_ARG;
+    std::vector<double> obs(size_t(LGF_NUM_OBS) * b->fields.size());
+    int rc = lgf_batch_measure_observables(b, flags, obs.data());
+    if (rc && rc != LGF_E_NAN) return rc;
+    for (size_t i = 0; i < b->fields.size(); ++i) {
+        if (E) E[i] = obs[LGF_NUM_OBS * i + LGF_OBS_E];
+        if (Q) Q[i] = obs[LGF_NUM_OBS * i + LGF_OBS_Q];
+        if (Qabs) Qabs[i] = obs[LGF_NUM_OBS * i + LGF_OBS_QABS];
+    }
+    return rc;
+}
+
+extern "C" int lgf_batch_measure_observables(lgf_batch* b, int flags, double* obs) {
+    if (!b || !obs) return LGF_E_ARG;
     lgf_ctx* ctx = b->ctx;
     int rc = batch_sync_desc(b);
     if (rc) return rc;
@@ -647,19 +673,22 @@ extern "C" int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, 
     for (const auto& c : chunks)
         HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, c.first, c.second, b->ping, ctx->lines,
                                   b->d_partials, ctx->stream));
-    HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials, 3, b->d_out, ctx->stream));
-    HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * 4 * n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials, kCloverParts, kCloverParts, b->d_out, ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * kOutStride * n, hipMemcpyDeviceToHost,
+                               ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
     bool finite = true;
+    const double rq = 1.0 / (4.0 * kPi * kPi);  // LGF/Observables/TopologicalCharge.cpp:21
     for (int i = 0; i < n; ++i) {
-        const double vol = (double)b->fields[i]->vol;
-        const double e = b->h_out[4 * i + 0] * (1.0 / vol);
-        const double q = b->h_out[4 * i + 1] * (1.0 / (4.0 * kPi * kPi));
-        const double qa = b->h_out[4 * i + 2] * (1.0 / (4.0 * kPi * kPi));
-        if (E) E[i] = e;
-        if (Q) Q[i] = q;
-        if (Qabs) Qabs[i] = qa;
-        finite = finite && std::isfinite(e) && std::isfinite(q);
+        const double rv = 1.0 / (double)b->fields[i]->vol;
+        const double* h = b->h_out + kOutStride * i;
+        double* o = obs + LGF_NUM_OBS * i;
+        o[LGF_OBS_E] = h[0] * rv;
+        o[LGF_OBS_Q] = h[1] * rq;
+        o[LGF_OBS_QABS] = h[2] * rq;
+        o[LGF_OBS_E_IMPROVED] = h[3] * rv;
+        o[LGF_OBS_Q_UNIMPROVED] = h[4] * rq;
+        for (int k = 0; k < LGF_NUM_OBS; ++k) finite = finite && std::isfinite(o[k]);
     }
     if (!finite) return lgf_fail(ctx, LGF_E_NAN, "non-finite energy density or charge");
     return LGF_OK;

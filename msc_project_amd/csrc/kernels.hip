@@ -58,19 +58,19 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8)
     const C eps = (C)d.eps;
     M3<C> E;
     if (STAGE == 0) {
-        store_link<S, C>(Xa, st, mu, s, Z);
+        store_ta<S, C>(Xa, st, mu, s, Z);
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
     } else if (STAGE == 1) {
-        M3<C> X0 = load_link<S, C>(Xa, st, mu, s);
+        M3<C> X0 = load_ta<S, C>(Xa, st, mu, s);
         M3<C> X1;
 #pragma unroll
         for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
-        store_link<S, C>(Xa, st, mu, s, X1);
+        store_ta<S, C>(Xa, st, mu, s, X1);
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
     } else {
-        M3<C> X1 = load_link<S, C>(Xa, st, mu, s);
+        M3<C> X1 = load_ta<S, C>(Xa, st, mu, s);
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
     }
@@ -199,13 +199,17 @@ LGF_DEV M3<C> project_F(const M3<C>& Cl) {
     return F;
 }
 
-// Plane (mu,nu): returns Re Tr(F11 F11) of the plain 1x1 F (energy density)
-// and the improved F = 19/9 F11 + 1/36 F22 - 32/45 F12 + 1/15 F13
-// (LGF/Field/SU3_field.cpp:588-597), where each F is the projection of its
-// clover average: 1/4 of four square leaves, 1/8 of eight rectangle leaves.
-// The projection is linear, so it is applied once to the weighted sum.
+// Plane (mu,nu): the plain 1x1 F (written packed to f11[0..8], one entry
+// every kTileSites doubles: LDS) and the improved
+// F = 19/9 F11 + 1/36 F22 - 32/45 F12 + 1/15 F13 (LGF/Field/SU3_field.cpp:
+// 588-597), where each F is the projection of its clover average: 1/4 of
+// four square leaves, 1/8 of eight rectangle leaves.  The projection is
+// linear, so it is applied once to the weighted sum.  Returns Re Tr(F11 F11).
+template <typename C>
+LGF_DEV void herm_pack(const M3<C>& F, double* h, int step);
+
 template <typename S, typename C>
-LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fimp) {
+LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fimp, double* f11) {
     PlaneOff o;
 #pragma unroll
     for (int k = -3; k <= 3; ++k) {
@@ -225,6 +229,7 @@ LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fim
     {
         const M3<C> F11 = project_F(acc);
         e = retr_mul(F11, F11);
+        herm_pack(F11, f11, kTileSites);
     }
     scale(acc, k1);
 #pragma unroll 1
@@ -255,16 +260,16 @@ LGF_DEV double herm_dot(const double* a, const double* b) {
 }
 
 template <typename C>
-LGF_DEV void herm_pack(const M3<C>& F, double* h) {
-    h[0] = (double)F.v[0];
-    h[1] = (double)F.v[8];
-    h[2] = (double)F.v[16];
-    h[3] = (double)F.v[2];
-    h[4] = (double)F.v[3];
-    h[5] = (double)F.v[4];
-    h[6] = (double)F.v[5];
-    h[7] = (double)F.v[10];
-    h[8] = (double)F.v[11];
+LGF_DEV void herm_pack(const M3<C>& F, double* h, int step) {
+    h[0 * step] = (double)F.v[0];
+    h[1 * step] = (double)F.v[8];
+    h[2 * step] = (double)F.v[16];
+    h[3 * step] = (double)F.v[2];
+    h[4 * step] = (double)F.v[3];
+    h[5 * step] = (double)F.v[4];
+    h[6 * step] = (double)F.v[5];
+    h[7 * step] = (double)F.v[10];
+    h[8 * step] = (double)F.v[11];
 }
 
 // L_2 and L_3 of one (site, direction) of a chunk of the batch's tiles
@@ -290,24 +295,31 @@ __global__ __launch_bounds__(256) void k_lines(const FieldDesc* __restrict__ fds
 // One thread per (site, plane).  The six planes are taken in the order of
 // the reference's charge, (3,2),(0,1) | (3,1),(2,0) | (3,0),(1,2)
 // (LGF/Observables/TopologicalCharge.cpp:12-16); they visit each plane
-// once, which the energy density sums over (EnergyDensity.cpp:13-20).
-// Per site: e(x) = sum_planes Re Tr(F11 F11), q(x) = sum_pairs Re Tr(Fa Fb)
-// with the improved F; tiles write (sum e, sum q, sum |q|) partials and,
-// when requested, q(x).  Reads the lines of the same chunk from k_lines.
+// once, which the energy densities sum over (EnergyDensity.cpp:13-20,
+// ImprovedEnergyDensity.cpp:13-20).  Per site:
+//   e(x)   = sum_planes Re Tr(F11 F11)            (EnergyDensity)
+//   q(x)   = sum_pairs  Re Tr(Fa Fb), improved F  (TopologicalCharge)
+//   ei(x)  = sum_planes Re Tr(Fimp Fimp)          (ImprovedEnergyDensity)
+//   q11(x) = sum_pairs  Re Tr(Fa Fb), 1x1 F       (UnImprovedTopologicalCharge,
+//                                                  UnImprovedTopologicalCharge.cpp:13-15)
+// Tiles write the kCloverParts partials (e, q, |q|, ei, q11) and, when
+// requested, q(x).  Reads the line products of the same chunk from k_lines.
+
 template <typename S, typename C>
 __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clover(
     const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int t0, int ping, const C* __restrict__ lines,
     int64_t G, double* __restrict__ partials) {
     __shared__ double sF[6][9][kTileSites];
+    __shared__ double sF11[6][9][kTileSites];
     __shared__ double sE[6][kTileSites];
+    __shared__ double sEi[6][kTileSites];
     const int tix = xcd_remap(blockIdx.x, gridDim.x);
     const int2 tm = tiles[tix];
     const FieldDesc& d = fds[tm.x];
     const int s = tm.y * kTileSites + threadIdx.x;
     const int p = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const bool live = s < d.vol;
-    double e = 0.0;
-    double h[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    double e = 0.0, ei = 0.0;
     if (live) {
         Lines<S, C> L;
         L.U = (const S*)d.links[ping ^ d.flip];
@@ -321,15 +333,18 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
         const int mu = (p & 1) ? (p == 1 ? 0 : (p == 3 ? 2 : 1)) : 3;
         const int nu = (p & 1) ? (p == 1 ? 1 : (p == 3 ? 0 : 2)) : (p == 0 ? 2 : (p == 2 ? 1 : 0));
         M3<C> Fi;
-        e = (double)plane_F<S, C>(L, g, mu, nu, Fi);
-        herm_pack(Fi, h);
-    }
+        e = (double)plane_F<S, C>(L, g, mu, nu, Fi, &sF11[p][0][threadIdx.x]);
+        ei = (double)retr_mul(Fi, Fi);
+        herm_pack(Fi, &sF[p][0][threadIdx.x], kTileSites);
+    } else {
 #pragma unroll
-    for (int i = 0; i < 9; ++i) sF[p][i][threadIdx.x] = h[i];
+        for (int i = 0; i < 9; ++i) sF[p][i][threadIdx.x] = sF11[p][i][threadIdx.x] = 0.0;
+    }
     sE[p][threadIdx.x] = e;
+    sEi[p][threadIdx.x] = ei;
     __syncthreads();
     if (p == 0) {
-        double a[9], b[9], qs = 0.0, es = 0.0;
+        double a[9], b[9], qs = 0.0, q11 = 0.0, es = 0.0, eis = 0.0;
 #pragma unroll 1
         for (int k = 0; k < 3; ++k) {
 #pragma unroll
@@ -338,9 +353,18 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
                 b[i] = sF[2 * k + 1][i][threadIdx.x];
             }
             qs += herm_dot(a, b);
+#pragma unroll
+            for (int i = 0; i < 9; ++i) {
+                a[i] = sF11[2 * k][i][threadIdx.x];
+                b[i] = sF11[2 * k + 1][i][threadIdx.x];
+            }
+            q11 += herm_dot(a, b);
         }
 #pragma unroll
-        for (int k = 0; k < 6; ++k) es += sE[k][threadIdx.x];
+        for (int k = 0; k < 6; ++k) {
+            es += sE[k][threadIdx.x];
+            eis += sEi[k][threadIdx.x];
+        }
         if (live && d.qx) d.qx[s] = qs;
         double qa = fabs(qs);
         // wavefront reduction (64 lanes), fixed order
@@ -349,12 +373,16 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
             es += __shfl_down(es, off, 64);
             qs += __shfl_down(qs, off, 64);
             qa += __shfl_down(qa, off, 64);
+            eis += __shfl_down(eis, off, 64);
+            q11 += __shfl_down(q11, off, 64);
         }
         if (threadIdx.x == 0) {
-            const int gt = d.tile0 + tm.y;
-            partials[3 * gt + 0] = es;
-            partials[3 * gt + 1] = qs;
-            partials[3 * gt + 2] = qa;
+            double* o = partials + kCloverParts * (d.tile0 + tm.y);
+            o[0] = es;
+            o[1] = qs;
+            o[2] = qa;
+            o[3] = eis;
+            o[4] = q11;
         }
     }
 }
@@ -362,7 +390,8 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
 // One workgroup per field: sum that field's tile partials (records of
 // `pstride` doubles, components 0..ncomp-1) in a fixed order, so that a
 // field's result does not depend on the other fields of its batch.
-// out[4*f + c]; for the clover partials c = {sum e, sum q, sum |q|}.
+// out[kOutStride*f + c]; for the clover partials c = {sum e, sum q, sum |q|,
+// sum ei, sum q11}.
 __global__ __launch_bounds__(256) void k_reduce(const FieldDesc* __restrict__ fds, const double* __restrict__ partials,
                                                 int pstride, int ncomp, double* __restrict__ out) {
     __shared__ double red[256];
@@ -376,7 +405,7 @@ __global__ __launch_bounds__(256) void k_reduce(const FieldDesc* __restrict__ fd
             if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
             __syncthreads();
         }
-        if (threadIdx.x == 0) out[4 * blockIdx.x + c] = red[0];
+        if (threadIdx.x == 0) out[kOutStride * blockIdx.x + c] = red[0];
         __syncthreads();
     }
 }
@@ -445,7 +474,7 @@ __global__ __launch_bounds__(256) void k_reduce_max(const FieldDesc* __restrict_
         if ((int)threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[4 * blockIdx.x + comp] = red[0];
+    if (threadIdx.x == 0) out[kOutStride * blockIdx.x + comp] = red[0];
 }
 
 // ---------------------------------------------------------------------------
@@ -581,9 +610,9 @@ hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2
     return hipGetLastError();
 }
 
-hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int ncomp, double* out,
+hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int pstride, int ncomp, double* out,
                          hipStream_t stream) {
-    hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(256), 0, stream, fds, partials, 3, ncomp, out);
+    hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(256), 0, stream, fds, partials, pstride, ncomp, out);
     return hipGetLastError();
 }
 

@@ -9,6 +9,8 @@ namespace lgf {
 
 constexpr int kPrecFP64 = 0;  // FP64 links, accumulator and arithmetic
 constexpr int kPrecFP32 = 1;  // FP32 links/arithmetic, FP64 reductions and reunitarisation
+constexpr int kCloverParts = 5;  // clover partials per tile: e, q, |q|, improved e, 1x1 q
+constexpr int kOutStride = 8;    // doubles per field in reduction outputs
 
 int num_stage_variants(int prec);
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
@@ -22,7 +24,9 @@ size_t line_bytes_per_site(int compute);
 // ntiles * 64 * line_bytes_per_site(compute) bytes
 hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2* tiles, int t0, int ntiles, int ping,
                          void* lines, double* partials, hipStream_t stream);
-hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int ncomp, double* out,
+// per-field sums of tile partials (records of pstride doubles, components
+// 0..ncomp-1) into out[kOutStride * field + component]
+hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int pstride, int ncomp, double* out,
                          hipStream_t stream);
 hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int ntiles, int ping, double* partials,
                        int nfields, double* out, hipStream_t stream);

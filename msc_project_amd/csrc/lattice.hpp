@@ -174,6 +174,56 @@ LGF_DEV void store_link(S* __restrict__ base, int64_t stride, int mu, int s, con
     }
 }
 
+// The RK3 accumulator holds traceless anti-Hermitian matrices whose lower
+// triangle is computed as -conj of the upper one and whose diagonal real
+// parts are exactly zero (su3.hpp, ta_scaled, and linear combinations of
+// such matrices), so 9 reals per link hold it exactly:
+//   a = (Im X00, Im X11, Im X22, Re X01, Im X01, Re X02, Im X02, Re X12, Im X12)
+// stored per direction mu in a block of 9*stride elements:
+//   FP32: two float4 planes (a0..a3, a4..a7) and one float plane (a8);
+//   FP64: four double2 planes and one double plane.
+template <typename S, typename C>
+LGF_DEV M3<C> load_ta(const S* __restrict__ base, int64_t stride, int mu, int s) {
+    const S* p = base + (int64_t)mu * 9 * stride;
+    C a[9];
+    if constexpr (sizeof(S) == 4) {
+        const vf4 v0 = ld<vf4>(p, (uint32_t)s * 16u), v1 = ld<vf4>(p + 4 * stride, (uint32_t)s * 16u);
+        a[0] = v0.x, a[1] = v0.y, a[2] = v0.z, a[3] = v0.w;
+        a[4] = v1.x, a[5] = v1.y, a[6] = v1.z, a[7] = v1.w;
+        a[8] = ld<float>(p + 8 * stride, (uint32_t)s * 4u);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const vd2 v = ld<vd2>(p + (int64_t)k * 2 * stride, (uint32_t)s * 16u);
+            a[2 * k] = (C)v.x;
+            a[2 * k + 1] = (C)v.y;
+        }
+        a[8] = (C)ld<double>(p + 8 * stride, (uint32_t)s * 8u);
+    }
+    M3<C> m;
+    m.v[0] = C(0), m.v[1] = a[0], m.v[8] = C(0), m.v[9] = a[1], m.v[16] = C(0), m.v[17] = a[2];
+    m.v[2] = a[3], m.v[3] = a[4], m.v[6] = -a[3], m.v[7] = a[4];      // (0,1), (1,0)
+    m.v[4] = a[5], m.v[5] = a[6], m.v[12] = -a[5], m.v[13] = a[6];    // (0,2), (2,0)
+    m.v[10] = a[7], m.v[11] = a[8], m.v[14] = -a[7], m.v[15] = a[8];  // (1,2), (2,1)
+    return m;
+}
+
+template <typename S, typename C>
+LGF_DEV void store_ta(S* __restrict__ base, int64_t stride, int mu, int s, const M3<C>& m) {
+    S* p = base + (int64_t)mu * 9 * stride;
+    if constexpr (sizeof(S) == 4) {
+        st_(p, (uint32_t)s * 16u, vf4{(float)m.v[1], (float)m.v[9], (float)m.v[17], (float)m.v[2]});
+        st_(p + 4 * stride, (uint32_t)s * 16u, vf4{(float)m.v[3], (float)m.v[4], (float)m.v[5], (float)m.v[10]});
+        st_(p + 8 * stride, (uint32_t)s * 4u, (float)m.v[11]);
+    } else {
+        st_(p, (uint32_t)s * 16u, vd2{(double)m.v[1], (double)m.v[9]});
+        st_(p + 2 * stride, (uint32_t)s * 16u, vd2{(double)m.v[17], (double)m.v[2]});
+        st_(p + 4 * stride, (uint32_t)s * 16u, vd2{(double)m.v[3], (double)m.v[4]});
+        st_(p + 6 * stride, (uint32_t)s * 16u, vd2{(double)m.v[5], (double)m.v[10]});
+        st_(p + 8 * stride, (uint32_t)s * 8u, (double)m.v[11]);
+    }
+}
+
 // Block -> tile mapping that keeps each XCD on a contiguous run of tiles.
 // Workgroups are dispatched to the 8 XCDs round-robin by blockIdx.x, and each
 // XCD has its own L2, so neighbouring tiles (which share halo links) should

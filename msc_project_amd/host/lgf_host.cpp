@@ -105,7 +105,7 @@ namespace {
 // per-field measurement cache: one clover sweep serves Q and E at one flow time
 struct MeasCache {
     uint64_t version = ~0ull;
-    double E = 0, Q = 0, Qabs = 0;
+    double obs[LGF_NUM_OBS] = {0, 0, 0, 0, 0};
 };
 std::map<const SU3_field*, uint64_t> g_version;
 std::map<const SU3_field*, MeasCache> g_cache;
@@ -175,17 +175,23 @@ void GradientFlow::MakeMeasurements() {
     if (m_EnergyDensity) m_EnergyDensity->calculate(m_flowTime);
 }
 
-void FieldMeasurement::measure(SU3_field& U, double, double* E, double* Q, double* Qabs) {
+void FieldMeasurement::observables(SU3_field& U, double* obs) {
     MeasCache& c = g_cache[&U];
     const uint64_t v = g_version[&U];
     if (c.version != v) {
-        int rc = lgf_measure(U.handle(), measure_flags(), &c.E, &c.Q, &c.Qabs, nullptr);
+        int rc = lgf_measure_observables(U.handle(), measure_flags(), c.obs, nullptr);
         if (rc != LGF_OK && rc != LGF_E_NAN) check(rc, "measure");  // NaN is printed like the reference does
         c.version = v;
     }
-    *E = c.E;
-    *Q = c.Q;
-    *Qabs = c.Qabs;
+    for (int k = 0; k < LGF_NUM_OBS; ++k) obs[k] = c.obs[k];
+}
+
+void FieldMeasurement::measure(SU3_field& U, double, double* E, double* Q, double* Qabs) {
+    double o[LGF_NUM_OBS];
+    observables(U, o);
+    *E = o[LGF_OBS_E];
+    *Q = o[LGF_OBS_Q];
+    *Qabs = o[LGF_OBS_QABS];
 }
 
 // ---------------------------------------------------------------------------
@@ -241,6 +247,43 @@ void TopologicalCharge::calculate(double flowTime) {
 void TopologicalCharge::saveTopologicalChargeToFile(double beta, const std::string& updateMethod,
                                                     const std::string& dataFolder, const std::string& identifier) {
     write_csv(observable_dir("Topological_Charge", beta, m_U->getLatticePtr().getShape(), updateMethod, dataFolder),
+              "torus_extdof4" + identifier, m_FlowMeasurementTimeVector, m_resultVector);
+}
+
+// LGF/Observables/ImprovedEnergyDensity.cpp:9-74: prints like EnergyDensity,
+// files under Observables/ImprovedEnergyDensity/
+void ImprovedEnergyDensity::calculate(double flowTime) {
+    double o[LGF_NUM_OBS];
+    FieldMeasurement::observables(*m_U, o);
+    m_resultVector.push_back(o[LGF_OBS_E_IMPROVED]);
+    m_FlowMeasurementTimeVector.push_back(flowTime);
+    std::cout << "Energy Density\t   = " << o[LGF_OBS_E_IMPROVED] << "\n";  // ImprovedEnergyDensity.cpp:29
+    std::cout.flush();
+}
+
+void ImprovedEnergyDensity::saveEnergyDensityToFile(double beta, const std::string& updateMethod,
+                                                    const std::string& dataFolder, const std::string& identifier) {
+    write_csv(observable_dir("ImprovedEnergyDensity", beta, m_U->getLatticePtr().getShape(), updateMethod,
+                             dataFolder),
+              "torus_extdof4" + identifier, m_FlowMeasurementTimeVector, m_resultVector);
+}
+
+// LGF/Observables/UnImprovedTopologicalCharge.cpp:8-67: the charge with the
+// 1x1 clover F, files under Observables/UnImprovedTopological_Charge/
+void UnImprovedTopologicalCharge::calculate(double flowTime) {
+    double o[LGF_NUM_OBS];
+    FieldMeasurement::observables(*m_U, o);
+    m_resultVector.push_back(o[LGF_OBS_Q_UNIMPROVED]);
+    m_FlowMeasurementTimeVector.push_back(flowTime);
+    std::cout << "Topological Charge = " << o[LGF_OBS_Q_UNIMPROVED] << "\n";  // UnImprovedTopologicalCharge.cpp:25
+    std::cout.flush();
+}
+
+void UnImprovedTopologicalCharge::saveTopologicalChargeToFile(double beta, const std::string& updateMethod,
+                                                              const std::string& dataFolder,
+                                                              const std::string& identifier) {
+    write_csv(observable_dir("UnImprovedTopological_Charge", beta, m_U->getLatticePtr().getShape(), updateMethod,
+                             dataFolder),
               "torus_extdof4" + identifier, m_FlowMeasurementTimeVector, m_resultVector);
 }
 
@@ -394,6 +437,27 @@ void MeasureQdensityDistribution(SimulationParameters& params, FlowParameters& f
         QdensityDistribution Qdensity(U, params.getbeta(), "GF", flowParams.getdataFolder(),
                                       ensembleNum + "_Flowtime" + std::to_string(flowTime_pickup));
         Qdensity.calculate(flowTime_pickup);
+    }
+}
+
+// LGF/LGF.cpp:228-255: unimproved Q and improved E of the flowed
+// configurations <id>_Flowtime<eps*flowSteps>, printed Q first, appended to
+// the CSVs torus_extdof4_<id>.csv
+void MeasureFlowedGaugeConfigurations(SimulationParameters& params, FlowParameters& flowParams) {
+    Lattice lattice(params, true);
+    double flowTime_pickup = flowParams.getepsilon() * flowParams.getflowSteps();
+    for (int i = flowParams.getConfigurationStart(); i <= flowParams.getConfigurationStop(); i++) {
+        SU3_field U(lattice, params.getextdofs());
+        std::string ensembleNum = std::to_string(i);
+        U.loadSU3FromFile(params.getbeta(), "GF", flowParams.getdataFolder(),
+                          ensembleNum + "_Flowtime" + std::to_string(flowTime_pickup));
+        UnImprovedTopologicalCharge unimprovedTopCharge(U);
+        unimprovedTopCharge.calculate(flowTime_pickup);
+        ImprovedEnergyDensity ImprovedEdensity(U);
+        ImprovedEdensity.calculate(flowTime_pickup);
+        unimprovedTopCharge.saveTopologicalChargeToFile(params.getbeta(), "GF", flowParams.getdataFolder(),
+                                                        "_" + ensembleNum);
+        ImprovedEdensity.saveEnergyDensityToFile(params.getbeta(), "GF", flowParams.getdataFolder(), "_" + ensembleNum);
     }
 }
 

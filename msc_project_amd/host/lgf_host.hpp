@@ -9,9 +9,12 @@
 //   Wilson                                   LGF/Action/Gauge/Wilson.h:7
 //   GradientFlow                             LGF/Gradient_Flow/GradientFlow.h:7-34
 //   EnergyDensity, TopologicalCharge         LGF/Observables/{EnergyDensity,TopologicalCharge}.h
+//   ImprovedEnergyDensity,                   LGF/Observables/ImprovedEnergyDensity.h,
+//   UnImprovedTopologicalCharge              LGF/Observables/UnImprovedTopologicalCharge.h
 //   QdensityDistribution                     LGF/Observables/QdensityDistribution.h
 //   FlowSavedGaugeConfigurations, MeasureQdensityDistribution,
-//   ResumeFlowedConfiguration, ParseCLargs   LGF/LGF.h:29-40
+//   ResumeFlowedConfiguration, MeasureFlowedGaugeConfigurations,
+//   ParseCLargs                              LGF/LGF.h:29-40
 // Differences, all deliberate:
 //   * errors throw lgf_host::Error instead of exit()/blocking on std::cin
 //     (LGF/SimulationParameters.h:12-19, LGF/Field/SU3_field.cpp:299-309);
@@ -174,6 +177,8 @@ private:
 class FieldMeasurement {
 public:
     static void measure(SU3_field& U, double flowTime, double* E, double* Q, double* Qabs);
+    // all lgf_observable values of the field's current links (obs[LGF_NUM_OBS])
+    static void observables(SU3_field& U, double* obs);
 };
 
 class EnergyDensity {
@@ -208,6 +213,34 @@ private:
     std::vector<double> m_FlowMeasurementTimeVector;
 };
 
+class ImprovedEnergyDensity {
+public:
+    explicit ImprovedEnergyDensity(SU3_field& U) : m_U(&U) {}
+    void calculate(double flowTime);
+    void saveEnergyDensityToFile(double beta, const std::string& updateMethod, const std::string& dataFolder,
+                                 const std::string& identifier = "");
+    const std::vector<double>& values() const { return m_resultVector; }
+
+private:
+    SU3_field* m_U;
+    std::vector<double> m_resultVector;
+    std::vector<double> m_FlowMeasurementTimeVector;
+};
+
+class UnImprovedTopologicalCharge {
+public:
+    explicit UnImprovedTopologicalCharge(SU3_field& U) : m_U(&U) {}
+    void calculate(double flowTime);
+    void saveTopologicalChargeToFile(double beta, const std::string& updateMethod, const std::string& dataFolder,
+                                     const std::string& identifier = "");
+    const std::vector<double>& values() const { return m_resultVector; }
+
+private:
+    SU3_field* m_U;
+    std::vector<double> m_resultVector;
+    std::vector<double> m_FlowMeasurementTimeVector;
+};
+
 class QdensityDistribution {
 public:
     QdensityDistribution(SU3_field& U, double beta, const std::string& updateMethod, const std::string& dataFolder,
@@ -223,6 +256,7 @@ private:
 void FlowSavedGaugeConfigurations(SimulationParameters& params, FlowParameters& flowParams);
 void MeasureQdensityDistribution(SimulationParameters& params, FlowParameters& flowParams);
 void ResumeFlowedConfiguration(SimulationParameters& params, FlowParameters& flowParams, double flowTime_pickup);
+void MeasureFlowedGaugeConfigurations(SimulationParameters& params, FlowParameters& flowParams);
 // FlowSavedGaugeConfigurations with all configurations of the range flowed
 // together in one batch (same results per configuration, same files).
 void FlowSavedGaugeConfigurationsBatched(SimulationParameters& params, FlowParameters& flowParams);

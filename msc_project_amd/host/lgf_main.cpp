@@ -7,6 +7,7 @@
 //   --mode=batch       the same, all configurations of the range in one batch
 //   --mode=qdensity    MeasureQdensityDistribution
 //   --mode=resume --pickup=T   ResumeFlowedConfiguration
+//   --mode=measured    MeasureFlowedGaugeConfigurations
 //   --precision=fp64|fp32      flow arithmetic (fp64 default)
 //   --measure=fp64|native      clover arithmetic of measurements (fp64 default)
 //   --input-method=NAME        input update-method directory (default LHMC_3_of_0_330000)
@@ -54,6 +55,7 @@ int main(int argc, char** argv) {
         else if (mode == "batch") FlowSavedGaugeConfigurationsBatched(params, flowParams);
         else if (mode == "qdensity") MeasureQdensityDistribution(params, flowParams);
         else if (mode == "resume") ResumeFlowedConfiguration(params, flowParams, pickup);
+        else if (mode == "measured") MeasureFlowedGaugeConfigurations(params, flowParams);
         else throw lgf_host::Error(LGF_E_ARG, "unknown mode " + mode);
     } catch (const lgf_host::Error& e) {
         std::cerr << "lgf: " << e.what() << "\n";

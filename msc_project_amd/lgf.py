@@ -20,6 +20,8 @@ FP64 = 0
 FP32 = 1
 MEASURE_DEFAULT = 0
 MEASURE_FP64 = 1
+# lgf_observable order (include/lgf_flow.h)
+OBSERVABLES = ("E", "Q", "Qabs", "E_improved", "Q_unimproved")
 
 STATUS = {0: "ok", 1: "bad argument", 2: "bad shape", 3: "I/O error", 4: "out of memory", 5: "HIP error",
           6: "non-finite result", 7: "invalid state", 8: "admission refused", 9: "unsupported"}
@@ -63,6 +65,8 @@ SIGNATURES = {
     "lgf_ctx_set_tile_order": ([_vp, ctypes.c_int], ctypes.c_int),
     "lgf_ctx_get_tile_order": ([_vp, _ip], ctypes.c_int),
     "lgf_ctx_set_measure_scratch": ([_vp, ctypes.c_size_t], ctypes.c_int),
+    "lgf_measure_observables": ([_vp, ctypes.c_int, _dp, _dp], ctypes.c_int),
+    "lgf_batch_measure_observables": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
     "lgf_field_footprint": ([_ip, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "lgf_field_create": ([_vp, _ip, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "lgf_field_destroy": ([_vp], ctypes.c_int),
@@ -276,6 +280,17 @@ class Field:
                                          _p(q) if want_q else None), "measure")
         return E.value, Q.value, Qa.value, q
 
+    def observables(self, flags=MEASURE_DEFAULT, want_q=False):
+        """All observables of one clover sweep as a dict (keys of OBSERVABLES),
+        plus "qraw" (per-site density, or None)."""
+        obs = np.empty(len(OBSERVABLES))
+        q = np.empty(self.vol) if want_q else None
+        self.ctx.check(lib().lgf_measure_observables(self.h, flags, _p(obs), _p(q) if want_q else None),
+                       "measure_observables")
+        out = dict(zip(OBSERVABLES, obs.tolist()))
+        out["qraw"] = q
+        return out
+
     def diagnostics(self):
         p, mx, mn = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self.ctx.check(lib().lgf_diagnostics(self.h, ctypes.byref(p), ctypes.byref(mx), ctypes.byref(mn)),
@@ -320,6 +335,12 @@ class Batch:
         E, Q, Qa = np.empty(n), np.empty(n), np.empty(n)
         self.ctx.check(lib().lgf_batch_measure(self.h, flags, _p(E), _p(Q), _p(Qa)), "batch measure")
         return E, Q, Qa
+
+    def observables(self, flags=MEASURE_DEFAULT):
+        """array (fields x len(OBSERVABLES)), columns in OBSERVABLES order"""
+        obs = np.empty((len(self.fields), len(OBSERVABLES)))
+        self.ctx.check(lib().lgf_batch_measure_observables(self.h, flags, _p(obs)), "batch measure_observables")
+        return obs
 
     def close(self):
         if self.h:

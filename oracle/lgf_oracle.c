@@ -21,6 +21,8 @@
  *                       clover2x2_avg, RectangleClover_avg,
  *                       RectangleClover1x3_avg, Improved_fieldStrengthTensor)
  *   orc_energy_density  LGF/Observables/EnergyDensity.cpp:9-34
+ *   orc_energy_density_kind  also ImprovedEnergyDensity.cpp:9-32
+ *   orc_top_charge_kind      also UnImprovedTopologicalCharge.cpp:8-29
  *   orc_top_charge      LGF/Observables/TopologicalCharge.cpp:8-29 and
  *                       LGF/Observables/QdensityDistribution.cpp:15-28
  *   orc_flow_run        LGF/LGF.cpp:163-196 measurement schedule
@@ -392,22 +394,24 @@ void orc_field_strength(const int *shape, const double *U, long g, int mu, int n
 /* ------------------------------------------------------------------ */
 /* observables                                                         */
 /* ------------------------------------------------------------------ */
-static double site_energy(const lat_t *L, const double *U, long g) {
+/* kind of F: 0 = 1x1 (EnergyDensity, UnImprovedTopologicalCharge),
+   4 = improved (TopologicalCharge, ImprovedEnergyDensity) */
+static double site_energy(const lat_t *L, const double *U, long g, int kind) {
     double F[MAT], s = 0;
     for (int mu = 0; mu < 4; ++mu)
         for (int nu = mu + 1; nu < 4; ++nu) {
-            orc_field_strength(L->n, U, g, mu, nu, 0, F);
+            orc_field_strength(L->n, U, g, mu, nu, kind, F);
             s += m_retr_mul(F, F);
         }
     return s;
 }
 
-static double site_charge(const lat_t *L, const double *U, long g) {
+static double site_charge(const lat_t *L, const double *U, long g, int kind) {
     static const int pr[3][4] = {{3, 2, 0, 1}, {3, 1, 2, 0}, {3, 0, 1, 2}};
     double Fa[MAT], Fb[MAT], s = 0;
     for (int p = 0; p < 3; ++p) {
-        orc_field_strength(L->n, U, g, pr[p][0], pr[p][1], 4, Fa);
-        orc_field_strength(L->n, U, g, pr[p][2], pr[p][3], 4, Fb);
+        orc_field_strength(L->n, U, g, pr[p][0], pr[p][1], kind, Fa);
+        orc_field_strength(L->n, U, g, pr[p][2], pr[p][3], kind, Fb);
         s += m_retr_mul(Fa, Fb);
     }
     return s;
@@ -425,11 +429,13 @@ static double internal_order_sum(const lat_t *L, const double *v) {
     return s;
 }
 
-double orc_energy_density(const int *shape, const double *U) {
+/* EnergyDensity (kind 0) or ImprovedEnergyDensity (kind 4,
+   LGF/Observables/ImprovedEnergyDensity.cpp:9-32) */
+double orc_energy_density_kind(const int *shape, const double *U, int kind) {
     lat_t L = mklat(shape);
     double *v = malloc(L.vol * sizeof(double));
 #pragma omp parallel for schedule(static)
-    for (long g = 0; g < L.vol; ++g) v[g] = site_energy(&L, U, g);
+    for (long g = 0; g < L.vol; ++g) v[g] = site_energy(&L, U, g, kind);
     double s = internal_order_sum(&L, v);
     free(v);
     return s * (1.0 / L.vol);
@@ -439,13 +445,18 @@ double orc_energy_density(const int *shape, const double *U) {
  * the reference's QdensityDistribution, i.e. scaled by (1.0/4.0*PI2) = pi^2/4
  * (LGF/Observables/QdensityDistribution.cpp:24), at global index g.
  * If abs_sum != NULL it receives sum_x |raw site sum| / (4 pi^2). */
-double orc_top_charge(const int *shape, const double *U, double *qx, double *abs_sum) {
+double orc_energy_density(const int *shape, const double *U) { return orc_energy_density_kind(shape, U, 0); }
+
+/* TopologicalCharge (kind 4) or UnImprovedTopologicalCharge (kind 0,
+   LGF/Observables/UnImprovedTopologicalCharge.cpp:8-29); qx and abs_sum as
+   for orc_top_charge */
+double orc_top_charge_kind(const int *shape, const double *U, int kind, double *qx, double *abs_sum) {
     static const double PI = 3.141592653589793238462643383279502884;
     double PI2 = PI * PI;
     lat_t L = mklat(shape);
     double *v = malloc(L.vol * sizeof(double));
 #pragma omp parallel for schedule(static)
-    for (long g = 0; g < L.vol; ++g) v[g] = site_charge(&L, U, g);
+    for (long g = 0; g < L.vol; ++g) v[g] = site_charge(&L, U, g, kind);
     double s = internal_order_sum(&L, v);
     if (qx)
         for (long g = 0; g < L.vol; ++g) qx[g] = (1.0 / 4.0 * PI2) * v[g];
@@ -456,6 +467,10 @@ double orc_top_charge(const int *shape, const double *U, double *qx, double *abs
     }
     free(v);
     return s * (1.0 / (4.0 * PI2));
+}
+
+double orc_top_charge(const int *shape, const double *U, double *qx, double *abs_sum) {
+    return orc_top_charge_kind(shape, U, 4, qx, abs_sum);
 }
 
 double orc_avg_plaquette(const int *shape, const double *U) {

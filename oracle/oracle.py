@@ -39,6 +39,10 @@ def lib():
         L.orc_energy_density.restype = ctypes.c_double
         L.orc_top_charge.argtypes = [_ip, _dp, _dp, _dp]
         L.orc_top_charge.restype = ctypes.c_double
+        L.orc_energy_density_kind.argtypes = [_ip, _dp, ctypes.c_int]
+        L.orc_energy_density_kind.restype = ctypes.c_double
+        L.orc_top_charge_kind.argtypes = [_ip, _dp, ctypes.c_int, _dp, _dp]
+        L.orc_top_charge_kind.restype = ctypes.c_double
         L.orc_avg_plaquette.argtypes = [_ip, _dp]
         L.orc_avg_plaquette.restype = ctypes.c_double
         L.orc_unitarity.argtypes = [_ip, _dp, _dp, _dp]
@@ -100,8 +104,14 @@ def field_strength(shape, U, g, mu, nu, kind):
     return out
 
 
-def energy_density(shape, U):
-    return lib().orc_energy_density(_shape(shape), _p(U))
+def energy_density(shape, U, improved=False):
+    """EnergyDensity, or ImprovedEnergyDensity when improved."""
+    return lib().orc_energy_density_kind(_shape(shape), _p(U), 4 if improved else 0)
+
+
+def top_charge_unimproved(shape, U):
+    """UnImprovedTopologicalCharge: Q with the 1x1 clover F."""
+    return lib().orc_top_charge_kind(_shape(shape), _p(U), 0, None, None)
 
 
 def top_charge(shape, U, want_qx=False):

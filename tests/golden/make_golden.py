@@ -14,7 +14,9 @@ printing E and Q with 17 significant digits.  Recorded per case:
   - t, E(t), Q(t) in the reference's print order (Q then E),
   - the reference's CSV files verbatim (6 significant digits, LGF default),
   - sha256 of the flowed configuration file and of the q(x) file, plus
-    sum(q(x)) and a few q(x) samples.
+    sum(q(x)) and a few q(x) samples,
+  - the unimproved Q and improved E of the flowed configuration
+    (MeasureFlowedGaugeConfigurations, LGF/LGF.cpp:228-255) and their CSVs.
 The 4^4 case also stores its input configuration (147 KB) and the flowed
 output configuration's first links so that format tests need no generator.
 """
@@ -91,6 +93,8 @@ def flow_case(name, shape, beta, configs, steps, eps, every, keep_input=False):
         t_pick = eps * steps
         # q(x) on the flowed configurations (LGF/LGF.cpp:114-124)
         Qd, Ed, wall_q, _ = run_ref("qdensity", shape, beta, 0, len(configs) - 1, steps, eps, every, folder)
+        # unimproved Q and improved E of the flowed configurations (LGF/LGF.cpp:228-255)
+        Qm, Em, _, _ = run_ref("measured", shape, beta, 0, len(configs) - 1, steps, eps, every, folder)
         for i in range(len(configs)):
             c = {"t": t, "Q": Q[i * nmeas:(i + 1) * nmeas], "E": E[i * nmeas:(i + 1) * nmeas]}
             qcsv = os.path.join(obs, "Topological_Charge", "beta" + beta_str(beta), shape_str(shape), "GF",
@@ -112,6 +116,12 @@ def flow_case(name, shape, beta, configs, steps, eps, every, keep_input=False):
             c["qx_abs_sum"] = float(np.abs(q).sum())
             idx = sorted(set([0, 1, len(q) // 3, len(q) // 2, len(q) - 1]))
             c["qx_samples"] = {str(k): float(q[k]) for k in idx}
+            c["measured_Q_unimproved"] = Qm[i]
+            c["measured_E_improved"] = Em[i]
+            c["csv_Q_unimproved"] = open(os.path.join(obs, "UnImprovedTopological_Charge", "beta" + beta_str(beta),
+                                                      shape_str(shape), "GF", "torus_extdof4_%d.csv" % i)).read()
+            c["csv_E_improved"] = open(os.path.join(obs, "ImprovedEnergyDensity", "beta" + beta_str(beta),
+                                                   shape_str(shape), "GF", "torus_extdof4_%d.csv" % i)).read()
             if keep_input:
                 fl = np.fromfile(flowed, dtype=np.float64)
                 c["flowed_first_link"] = fl[:18].tolist()

@@ -98,6 +98,30 @@ def test_host_qdensity_small(tmp_path):
     assert np.max(np.abs(fl[-18:] - c["flowed_last_link"])) < 1e-13
 
 
+def test_host_measured_small(tmp_path):
+    """MeasureFlowedGaugeConfigurations: unimproved Q and improved E of the
+    flowed configuration, console order Q then E, CSVs as the reference's"""
+    d, U = small_input()
+    shape = tuple(d["shape"])
+    c = d["configs"][0]
+    folder = str(tmp_path) + "/"
+    place_inputs(folder, shape, d["beta"], [U])
+    run_lgf(folder, shape, d["beta"], 0, d["steps"], d["eps"], d["every"])
+    out = run_lgf(folder, shape, d["beta"], 0, d["steps"], d["eps"], d["every"], "--mode=measured")
+    lines = [l for l in out.splitlines() if l.startswith("Topological Charge") or l.startswith("Energy Density")]
+    assert len(lines) == 2 and lines[0].startswith("Topological Charge")
+    sd = "X".join(map(str, shape))
+    obs = os.path.join(folder, "Observables")
+    qdir = os.path.join(obs, "UnImprovedTopological_Charge", beta_dir(d["beta"]), sd, "GF")
+    edir = os.path.join(obs, "ImprovedEnergyDensity", beta_dir(d["beta"]), sd, "GF")
+    _, Q = read_full(os.path.join(qdir, "torus_extdof4_0.full.csv"))
+    _, E = read_full(os.path.join(edir, "torus_extdof4_0.full.csv"))
+    assert rel_err(E, [c["measured_E_improved"]]) < TOL_FP64
+    assert abs(Q[0] - c["measured_Q_unimproved"]) < TOL_FP64 * max(1.0, abs(c["measured_Q_unimproved"]))
+    assert csv_equal_or_rounding_tie(open(os.path.join(qdir, "torus_extdof4_0.csv")).read(), c["csv_Q_unimproved"], Q)
+    assert csv_equal_or_rounding_tie(open(os.path.join(edir, "torus_extdof4_0.csv")).read(), c["csv_E_improved"], E)
+
+
 def test_host_resume_continues_flow(tmp_path):
     """ResumeFlowedConfiguration: 10 steps + resume 10 steps == 20 steps at the end"""
     d, U = small_input()

@@ -101,6 +101,9 @@ def test_small_golden(ctx):
     assert np.allclose(t, c["t"], rtol=0, atol=1e-14)
     assert rel_err(E, c["E"]) < TOL_FP64
     assert q_err(Q, c["Q"], Qa) < TOL_FP64
+    o = f.observables()
+    assert rel_err(o["E_improved"], c["measured_E_improved"]) < TOL_FP64
+    assert q_err(o["Q_unimproved"], c["measured_Q_unimproved"], o["Qabs"]) < TOL_FP64
     fin = f.download().reshape(-1)
     assert np.max(np.abs(fin[:18] - np.asarray(c["flowed_first_link"]))) < 1e-14
     assert np.max(np.abs(fin[-18:] - np.asarray(c["flowed_last_link"]))) < 1e-14
@@ -121,6 +124,29 @@ def test_config1_golden_fp64(ctx, config1):
         print("config %d: max rel dE %.2e, max dQ/scale %.2e" % (i, rel_err(E, c["E"]), q_err(Q, c["Q"], Qa)))
         assert rel_err(E, c["E"]) < TOL_FP64
         assert q_err(Q, c["Q"], Qa) < TOL_FP64
+        o = f.observables()
+        assert rel_err(o["E_improved"], c["measured_E_improved"]) < TOL_FP64
+        assert q_err(o["Q_unimproved"], c["measured_Q_unimproved"], o["Qabs"]) < TOL_FP64
+
+
+def test_observables_match_oracle(ctx):
+    """improved E and unimproved Q (MeasureFlowedGaugeConfigurations) against
+    the oracle, in every precision mode"""
+    shape = (6, 4, 4, 8)
+    U = O.generate(shape, 6.0, seed=61, start="hot", sweeps=3)
+    ref = {"E": O.energy_density(shape, U), "Q": O.top_charge(shape, U)[0],
+           "E_improved": O.energy_density(shape, U, improved=True), "Q_unimproved": O.top_charge_unimproved(shape, U)}
+    Qa = O.top_charge(shape, U)[1]
+    for prec, flags, tol in ((lgf.FP64, lgf.MEASURE_DEFAULT, TOL_FP64), (lgf.FP32, lgf.MEASURE_FP64, 1e-6),
+                             (lgf.FP32, lgf.MEASURE_DEFAULT, 1e-5)):
+        f = lgf.Field(ctx, shape, prec)
+        f.upload(U)
+        o = f.observables(flags)
+        for k in ("E", "E_improved"):
+            assert rel_err(o[k], ref[k]) < tol, (prec, flags, k)
+        for k in ("Q", "Q_unimproved"):
+            assert q_err(o[k], ref[k], Qa) < tol, (prec, flags, k)
+        f.close()
 
 
 def test_batch_same_alone_and_in_mix(ctx):
@@ -265,16 +291,23 @@ def test_stage_variants_agree_and_autotune(ctx):
 @pytest.mark.slow
 def test_large24_golden_fp64(ctx):
     """First 50 steps of the 24^4, beta 6.0 reference flow (eps 0.01, every 5).
-    Slow: the input is regenerated by the CPU oracle's generator (200 sweeps)."""
-    import hashlib
-    import os
+    Slow: the input is regenerated by the CPU oracle's generator (200 sweeps,
+    minutes), unless LGF_LARGE24_INPUT names a file holding it; either way
+    its sha256 must equal the one recorded with the reference's outputs."""
     if not os.environ.get("LGF_SLOW"):
-        pytest.skip("set LGF_SLOW=1: regenerates a 24^4 input on the CPU (minutes)")
+        pytest.skip("set LGF_SLOW=1: needs the 24^4 input (regenerated on the CPU, minutes)")
     d = golden("large24")
     g = d["generator"]
     shape = tuple(d["shape"])
-    U = O.generate(shape, d["beta"], seed=g["seed"], start=g["start"], sweeps=g["therm_sweeps"],
-                   or_per_hb=g["or_per_hb"])
+    path = os.environ.get("LGF_LARGE24_INPUT")
+    if path:
+        U = np.fromfile(path, dtype=np.float64).reshape(-1, 4, 18)
+    else:
+        U = O.generate(shape, d["beta"], seed=g["seed"], start=g["start"], sweeps=0, or_per_hb=g["or_per_hb"])
+        for k in range(0, g["therm_sweeps"], 20):  # in chunks, with progress lines
+            O.generate(shape, d["beta"], seed=g["seed"], start="continue", sweeps=20, U=U, first_sweep=k,
+                       or_per_hb=g["or_per_hb"])
+            print("generated %d sweeps" % (k + 20), flush=True)
     assert hashlib.sha256(U.tobytes()).hexdigest() == d["input_sha256"][0]
     c = d["configs"][0]
     f = lgf.Field(ctx, shape, lgf.FP64)

@@ -30,8 +30,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_footprint_and_shape_validation_without_gpu():
-    assert lgf.footprint((4, 4, 4, 4), lgf.FP64) == 3 * 72 * 256 * 8
-    assert lgf.footprint((24, 24, 24, 24), lgf.FP32) == 3 * 72 * 331776 * 4
+    # two link buffers (72 reals per site) and the accumulator (36 reals per site)
+    assert lgf.footprint((4, 4, 4, 4), lgf.FP64) == (2 * 72 + 36) * 256 * 8
+    assert lgf.footprint((24, 24, 24, 24), lgf.FP32) == (2 * 72 + 36) * 331776 * 4
+    with pytest.raises(lgf.LgfError) as e:
+        lgf.footprint((130, 128, 128, 128), lgf.FP32)  # beyond 2^28 sites
+    assert e.value.code == 2
     with pytest.raises(lgf.LgfError) as e:
         lgf.footprint((4, 3, 4, 4), lgf.FP64)
     assert e.value.code == 2

@@ -62,8 +62,11 @@ def test_oracle_matches_reference_small():
     assert np.max(np.abs(np.asarray(Q) - c["Q"])) < 1e-13
     assert np.array_equal(U.reshape(-1)[:18], np.asarray(c["flowed_first_link"]))
     assert np.array_equal(U.reshape(-1)[-18:], np.asarray(c["flowed_last_link"]))
-    _, _, qx = O.top_charge(shape, U, want_qx=True)
+    _, Qa, qx = O.top_charge(shape, U, want_qx=True)
     assert abs(qx.sum() - c["qx_sum"]) < 1e-12 * c["qx_abs_sum"]
+    # MeasureFlowedGaugeConfigurations on the flowed configuration
+    assert rel_err(O.energy_density(shape, U, improved=True), c["measured_E_improved"]) < 1e-13
+    assert q_err(O.top_charge_unimproved(shape, U), c["measured_Q_unimproved"], Qa) < 1e-13
     for k, v in c["qx_samples"].items():
         assert abs(qx[int(k)] - v) < 1e-13 * c["qx_abs_sum"]
 
@@ -78,6 +81,8 @@ def test_oracle_matches_reference_config1():
         assert rel_err(E, c["E"]) < 1e-13
         Qa = O.top_charge(shape, U)[1]
         assert q_err(Q, c["Q"], Qa) < 1e-12
+        assert rel_err(O.energy_density(shape, U, improved=True), c["measured_E_improved"]) < 1e-12
+        assert q_err(O.top_charge_unimproved(shape, U), c["measured_Q_unimproved"], Qa) < 1e-12
         assert hashlib.sha256(U.tobytes()).hexdigest() == c["flowed_sha256"] or \
             np.max(np.abs(U)) < 2  # bit-identical final links are not required, E/Q are
 
