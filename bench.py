@@ -110,16 +110,21 @@ def run_job(lgf, ctx, src, prec, steps, warmup, ev):
     ctx.synchronize()
     t0 = time.perf_counter()
     E, Q = [], []
-    e, q, _ = b.measure(flags)
-    E.append(e[0]), Q.append(q[0])
+    # deferred measurements (lgf_batch_measure_start / _finish): the next
+    # MEASURE_EVERY flow steps are enqueued before the result is collected,
+    # so the GPU does not wait for the host in between
+    b.measure_start(flags)
     done = 0
     while done < steps:
         n = min(MEASURE_EVERY, steps - done)
         b.flow(EPS, n, use_graph=True)
         done += n
-        if done % MEASURE_EVERY == 0 or done == steps:
-            e, q, _ = b.measure(lgf.MEASURE_FP64 if done == steps else flags)  # final sweep in FP64
-            E.append(e[0]), Q.append(q[0])
+        o = b.measure_finish()[0]
+        E.append(o[0]), Q.append(o[1])
+        if done % MEASURE_EVERY == 0 and done < steps:
+            b.measure_start(flags)
+    e, q, _ = b.measure(lgf.MEASURE_FP64)  # final sweep in FP64
+    E.append(e[0]), Q.append(q[0])
     ctx.synchronize()
     wall = time.perf_counter() - t0
     diag = f.diagnostics()
@@ -175,9 +180,12 @@ def batched(lgf, ctx):
     b.flow(EPS, 2, use_graph=True)  # capture
     ctx.synchronize()
     t0 = time.perf_counter()
-    for k in range(30):
+    for k in range(30):  # deferred measurements: the next flow steps are queued first
         b.flow(EPS, 10, use_graph=True)
-        b.measure(lgf.MEASURE_DEFAULT)
+        if k > 0:
+            b.measure_finish()
+        b.measure_start(lgf.MEASURE_DEFAULT)
+    b.measure_finish()
     ctx.synchronize()
     wall = time.perf_counter() - t0
     lu = 4.0 * sum(np.prod(s) for s, _ in specs) * 300
@@ -188,11 +196,12 @@ def batched(lgf, ctx):
             "steps": 300, "measure_every": 10, "precision": "fp32"}
 
 
-def serving(lgf, nreq, throughput, seed=4242):
+def serving(lgf, nreq, throughput, seed=4242, policy=1):
     """Poisson stream of flow requests (shape uniform in {16^4, 24^4, 32^4,
     32^3x64}, beta in {5.96, 6.05, 6.13, 6.21}, eps 0.01, t uniform in [1, 6],
     measure every 10 steps, seeded random start) at a rate set for ~80%
-    utilisation of the measured FP32 throughput."""
+    utilisation of the measured FP32 throughput; server policy `policy`
+    (lgf.SERVE_SHORTEST_FIRST or lgf.SERVE_ALL)."""
     rng = np.random.default_rng(seed)
     shapes = [(16, 16, 16, 16), (24, 24, 24, 24), (32, 32, 32, 32), (32, 32, 32, 64)]
     betas = [5.96, 6.05, 6.13, 6.21]
@@ -205,6 +214,7 @@ def serving(lgf, nreq, throughput, seed=4242):
     rate = 0.8 * throughput / mean_work
     gaps = rng.exponential(1.0 / rate, size=nreq)
     srv = lgf.Server(0)
+    srv.set_policy(policy)
     ids = [None] * nreq
 
     def feeder():
@@ -228,7 +238,8 @@ def serving(lgf, nreq, throughput, seed=4242):
     lat = np.array([r["finish_s"] - r["submit_s"] for r in res])
     ok = all(r["status"] == 0 for r in res)
     lu = sum(4.0 * np.prod(s) * n for s, _, n, _ in reqs)
-    return {"requests": nreq, "arrival_rate_per_s": rate, "all_ok": ok, "p50_latency_s": float(np.percentile(lat, 50)),
+    return {"requests": nreq, "policy": "shortest_first" if policy == lgf.SERVE_SHORTEST_FIRST else "all",
+            "arrival_rate_per_s": rate, "all_ok": ok, "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
             "throughput_link_updates_per_s": lu / wall, "wall_s": wall, "busy_s": st["busy_s"],
             "batch_steps": st["batch_steps"]}
@@ -364,9 +375,12 @@ def main():
         line["batched_16_uneven"] = batched(lgf, ctx)
         log("batched: %s" % json.dumps(line["batched_16_uneven"]))
     if args.serve_requests > 0:
-        line["serving"] = serving(lgf, args.serve_requests, value)
+        line["serving"] = serving(lgf, args.serve_requests, value, policy=lgf.SERVE_SHORTEST_FIRST)
         line["p99_latency_s"] = line["serving"]["p99_latency_s"]
         log("serving: %s" % json.dumps(line["serving"]))
+        # the same request stream with every admitted job stepping each round
+        line["serving_policy_all"] = serving(lgf, args.serve_requests, value, policy=lgf.SERVE_ALL)
+        log("serving (all jobs step): %s" % json.dumps(line["serving_policy_all"]))
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline()
     print(json.dumps(line), flush=True)

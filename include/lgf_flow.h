@@ -166,6 +166,17 @@ int lgf_batch_flow_steps(lgf_batch* b, const double* eps, int nsteps, int use_gr
 int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, double* Qabs);
 /* obs[LGF_NUM_OBS * field + lgf_observable] */
 int lgf_batch_measure_observables(lgf_batch* b, int flags, double* obs);
+/* deferred measurement: _start enqueues the clover sweep of the current
+ * links (after all work enqueued on the context so far) and returns at once,
+ * so flow steps can be enqueued behind it; _finish waits and returns the
+ * observables of that flow time (as lgf_batch_measure_observables).  One
+ * measurement per batch may be outstanding (else LGF_E_STATE).  No
+ * counterpart in the reference, which measures between flow steps
+ * (LGF/Gradient_Flow/GradientFlow.cpp:52-58). */
+int lgf_batch_measure_start(lgf_batch* b, int flags);
+int lgf_batch_measure_finish(lgf_batch* b, double* obs);
+int lgf_measure_start(lgf_field* f, int flags);
+int lgf_measure_finish(lgf_field* f, double* obs);
 
 /* ---- continuous-batching flow server ----------------------------------- */
 typedef struct {
@@ -198,6 +209,15 @@ int lgf_server_destroy(lgf_server* s);
 int lgf_server_submit(lgf_server* s, const lgf_request* req, uint64_t* id);
 /* wait (timeout_s < 0: forever) for the job; LGF_E_STATE on timeout */
 int lgf_server_wait(lgf_server* s, uint64_t id, double timeout_s, lgf_result* out);
+/* scheduling policy.  LGF_SERVE_ALL: every admitted job steps in every
+ * round (processor sharing).  LGF_SERVE_SHORTEST_FIRST (default): the
+ * admitted jobs with the least remaining link-updates step, in that order,
+ * until they hold min_sites lattice sites (default 2^20), so short jobs
+ * finish first while each launch still fills the GPU; the choice is
+ * revisited whenever a job arrives or finishes.  Results do not depend on
+ * the policy. */
+enum lgf_serve_policy { LGF_SERVE_ALL = 0, LGF_SERVE_SHORTEST_FIRST = 1 };
+int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites);
 int lgf_server_now(lgf_server* s, double* now_s);
 /* counters: jobs done, steps executed, link-updates executed, busy seconds */
 int lgf_server_stats(lgf_server* s, uint64_t* jobs_done, uint64_t* batch_steps, double* link_updates,

@@ -531,6 +531,10 @@ extern "C" int lgf_batch_destroy(lgf_batch* b) {
     if (b->d_partials) (void)hipFree(b->d_partials);
     if (b->d_out) (void)hipFree(b->d_out);
     if (b->h_out) (void)hipHostFree(b->h_out);
+    if (b->d_partials_m) (void)hipFree(b->d_partials_m);
+    if (b->d_out_m) (void)hipFree(b->d_out_m);
+    if (b->h_out_m) (void)hipHostFree(b->h_out_m);
+    if (b->ev_done) (void)hipEventDestroy(b->ev_done);
     delete b;
     return LGF_OK;
 }
@@ -639,49 +643,47 @@ extern "C" int lgf_batch_measure(lgf_batch* b, int flags, double* E, double* Q, 
     return rc;
 }
 
-extern "C" int lgf_batch_measure_observables(lgf_batch* b, int flags, double* obs) {
-    if (!b || !obs) return LGF_E_ARG;
-    lgf_ctx* ctx = b->ctx;
-    int rc = batch_sync_desc(b);
-    if (rc) return rc;
-    const int compute = (flags & LGF_MEASURE_FP64) ? LGF_FP64 : b->prec;
-    const int n = (int)b->fields.size();
-    // consecutive fields are measured together while their line products
-    // fit ctx->lines_cap (a single larger field is measured on its own)
+// Groups of consecutive fields whose line products fit ctx->lines_cap (a
+// single larger field forms a group of its own): (first tile, tiles), and
+// the line-product bytes the largest group needs.
+static std::vector<std::pair<int, int>> measure_chunks(const lgf_batch* b, int compute, size_t* need) {
     const size_t per_tile = line_bytes_per_site(compute) * kTileSites;
-    std::vector<std::pair<int, int>> chunks;  // (first tile, tiles)
-    size_t need = 0;
+    const int n = (int)b->fields.size();
+    std::vector<std::pair<int, int>> chunks;
+    *need = 0;
     for (int i = 0; i < n;) {
         int t0 = b->h_desc[i].tile0, nt = 0;
-        while (i < n && (nt == 0 || size_t(nt + b->h_desc[i].ntiles) * per_tile <= ctx->lines_cap))
+        while (i < n && (nt == 0 || size_t(nt + b->h_desc[i].ntiles) * per_tile <= b->ctx->lines_cap))
             nt += b->h_desc[i++].ntiles;
         chunks.emplace_back(t0, nt);
-        need = std::max(need, size_t(nt) * per_tile);
+        *need = std::max(*need, size_t(nt) * per_tile);
     }
-    if (need > ctx->lines_bytes) {
-        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
-        if (ctx->lines) (void)hipFree(ctx->lines);
-        ctx->lines = nullptr;
-        ctx->lines_bytes = 0;
-        if (hipMalloc(&ctx->lines, need) != hipSuccess) {
-            (void)hipGetLastError();
-            ctx->lines = nullptr;
-            return lgf_fail(ctx, LGF_E_NOMEM, "no device memory for the clover line products");
-        }
-        ctx->lines_bytes = need;
+    return chunks;
+}
+
+// device buffer of at least `need` bytes, grown (after a wait on `stream`)
+static int grow_buffer(lgf_ctx* ctx, void** buf, size_t* have, size_t need, hipStream_t stream, const char* what) {
+    if (need <= *have) return LGF_OK;
+    HIPTRY(ctx, hipStreamSynchronize(stream));
+    if (*buf) (void)hipFree(*buf);
+    *buf = nullptr;
+    *have = 0;
+    if (hipMalloc(buf, need) != hipSuccess) {
+        (void)hipGetLastError();
+        *buf = nullptr;
+        return lgf_fail(ctx, LGF_E_NOMEM, std::string("no device memory for ") + what);
     }
-    for (const auto& c : chunks)
-        HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, c.first, c.second, b->ping, ctx->lines,
-                                  b->d_partials, ctx->stream));
-    HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials, kCloverParts, kCloverParts, b->d_out, ctx->stream));
-    HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * kOutStride * n, hipMemcpyDeviceToHost,
-                               ctx->stream));
-    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    *have = need;
+    return LGF_OK;
+}
+
+// per-field reduction records -> lgf_observable values
+static int observables_from(lgf_batch* b, const double* h_out, double* obs) {
     bool finite = true;
     const double rq = 1.0 / (4.0 * kPi * kPi);  // LGF/Observables/TopologicalCharge.cpp:21
-    for (int i = 0; i < n; ++i) {
+    for (size_t i = 0; i < b->fields.size(); ++i) {
         const double rv = 1.0 / (double)b->fields[i]->vol;
-        const double* h = b->h_out + kOutStride * i;
+        const double* h = h_out + kOutStride * i;
         double* o = obs + LGF_NUM_OBS * i;
         o[LGF_OBS_E] = h[0] * rv;
         o[LGF_OBS_Q] = h[1] * rq;
@@ -690,8 +692,87 @@ extern "C" int lgf_batch_measure_observables(lgf_batch* b, int flags, double* ob
         o[LGF_OBS_Q_UNIMPROVED] = h[4] * rq;
         for (int k = 0; k < LGF_NUM_OBS; ++k) finite = finite && std::isfinite(o[k]);
     }
-    if (!finite) return lgf_fail(ctx, LGF_E_NAN, "non-finite energy density or charge");
+    if (!finite) return lgf_fail(b->ctx, LGF_E_NAN, "non-finite energy density or charge");
     return LGF_OK;
+}
+
+extern "C" int lgf_batch_measure_observables(lgf_batch* b, int flags, double* obs) {
+    if (!b || !obs) return LGF_E_ARG;
+    lgf_ctx* ctx = b->ctx;
+    int rc = batch_sync_desc(b);
+    if (rc) return rc;
+    const int compute = (flags & LGF_MEASURE_FP64) ? LGF_FP64 : b->prec;
+    const int n = (int)b->fields.size();
+    size_t need;
+    const auto chunks = measure_chunks(b, compute, &need);
+    rc = grow_buffer(ctx, &ctx->lines, &ctx->lines_bytes, need, ctx->stream, "the clover line products");
+    if (rc) return rc;
+    for (const auto& c : chunks)
+        HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, c.first, c.second, b->ping, ctx->lines,
+                                  b->d_partials, ctx->stream));
+    HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials, kCloverParts, kCloverParts, b->d_out, ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * kOutStride * n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    return observables_from(b, b->h_out, obs);
+}
+
+// Deferred measurement.  The sweep is enqueued on the context's stream after
+// all work enqueued so far, so it sees the links of the current flow time;
+// flow steps enqueued afterwards follow it in stream order, and the caller
+// collects the result later, so the stream never waits for the host between
+// a measurement and the next flow steps.  One measurement may be outstanding
+// per batch; it has its own partials and result buffers.
+extern "C" int lgf_batch_measure_start(lgf_batch* b, int flags) {
+    if (!b) return LGF_E_ARG;
+    lgf_ctx* ctx = b->ctx;
+    if (b->pending) return lgf_fail(ctx, LGF_E_STATE, "a measurement of this batch is still outstanding");
+    int rc = batch_sync_desc(b);
+    if (rc) return rc;
+    const int n = (int)b->fields.size();
+    if (!b->ev_done) {
+        HIPTRY(ctx, hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming));
+        hipError_t e = hipMalloc(&b->d_partials_m, sizeof(double) * kCloverParts * b->ntiles);
+        if (e == hipSuccess) e = hipMalloc(&b->d_out_m, sizeof(double) * kOutStride * n);
+        if (e == hipSuccess) e = hipHostMalloc((void**)&b->h_out_m, sizeof(double) * kOutStride * n, hipHostMallocDefault);
+        if (e != hipSuccess) return lgf_hip_fail(ctx, e, "measurement buffers");
+    }
+    const int compute = (flags & LGF_MEASURE_FP64) ? LGF_FP64 : b->prec;
+    size_t need;
+    const auto chunks = measure_chunks(b, compute, &need);
+    rc = grow_buffer(ctx, &ctx->lines, &ctx->lines_bytes, need, ctx->stream, "the clover line products");
+    if (rc) return rc;
+    for (const auto& c : chunks)
+        HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, c.first, c.second, b->ping, ctx->lines,
+                                  b->d_partials_m, ctx->stream));
+    HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials_m, kCloverParts, kCloverParts, b->d_out_m, ctx->stream));
+    HIPTRY(ctx, hipMemcpyAsync(b->h_out_m, b->d_out_m, sizeof(double) * kOutStride * n, hipMemcpyDeviceToHost,
+                               ctx->stream));
+    HIPTRY(ctx, hipEventRecord(b->ev_done, ctx->stream));
+    b->pending = true;
+    return LGF_OK;
+}
+
+extern "C" int lgf_batch_measure_finish(lgf_batch* b, double* obs) {
+    if (!b || !obs) return LGF_E_ARG;
+    lgf_ctx* ctx = b->ctx;
+    if (!b->pending) return lgf_fail(ctx, LGF_E_STATE, "no measurement of this batch is outstanding");
+    b->pending = false;
+    HIPTRY(ctx, hipEventSynchronize(b->ev_done));
+    return observables_from(b, b->h_out_m, obs);
+}
+
+extern "C" int lgf_measure_start(lgf_field* f, int flags) {
+    if (!f) return LGF_E_ARG;
+    lgf_batch* b;
+    int rc = solo(f, &b);
+    if (rc) return rc;
+    return lgf_batch_measure_start(b, flags);
+}
+
+extern "C" int lgf_measure_finish(lgf_field* f, double* obs) {
+    if (!f || !f->solo) return LGF_E_ARG;
+    return lgf_batch_measure_finish(f->solo, obs);
 }
 
 // ---------------------------------------------------------------------------

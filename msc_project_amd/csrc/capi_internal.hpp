@@ -55,6 +55,13 @@ struct lgf_batch {
     hipGraphExec_t graph[2] = {nullptr, nullptr};  // two RK3 steps, by starting ping
     int graph_variant[2] = {-1, -1};
     bool is_solo = false;  // a field's private one-field batch
+    // deferred measurement (lgf_batch_measure_start / _finish): own partials
+    // and result buffers, completion event
+    hipEvent_t ev_done = nullptr;
+    double* d_partials_m = nullptr;
+    double* d_out_m = nullptr;
+    double* h_out_m = nullptr;
+    bool pending = false;
 };
 
 int lgf_fail(lgf_ctx* ctx, int code, const std::string& msg);

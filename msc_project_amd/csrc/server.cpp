@@ -5,10 +5,15 @@
 // time; one scheduler thread owns the GPU context and loops:
 //   1. admit queued jobs, first come first served, while their device
 //      footprint (2 link buffers + RK3 accumulator) fits the budget;
-//   2. run k RK3 steps for every active job with one set of launches per
-//      precision (a batch over all active lattices, whatever their shapes;
-//      k = steps to the nearest measurement or completion, at most kMaxGroup,
-//      so new arrivals join within a few steps);
+//   2. choose the jobs that step: every admitted job (LGF_SERVE_ALL), or
+//      (LGF_SERVE_SHORTEST_FIRST) the admitted jobs with the least remaining
+//      link-updates, in that order, until they hold min_sites lattice sites,
+//      so short jobs finish first while each launch still fills the GPU; the
+//      choice is revisited when a job arrives or finishes;
+//      run k RK3 steps for those jobs with one set of launches per precision
+//      (a batch over their lattices, whatever their shapes; k = steps to the
+//      nearest measurement or completion, at most kMaxGroup, so new arrivals
+//      are seen within a few steps);
 //   3. measure the jobs that are due, retire finished jobs and free their
 //      memory, which lets the next queued jobs in.
 // Every job's arithmetic is independent of the other jobs in the batch, and
@@ -56,6 +61,8 @@ struct lgf_server {
     int device = 0;
     size_t budget = 0;
     int max_active = 0;
+    int policy = LGF_SERVE_SHORTEST_FIRST;  // guarded by mu
+    int64_t min_sites = int64_t(1) << 20;
     lgf_ctx* ctx = nullptr;
     std::thread thread;
     std::mutex mu;
@@ -97,13 +104,16 @@ static void measure_job(lgf_server* S, Job& j) {
     }
 }
 
+static double remaining_work(const Job& j) { return 4.0 * (double)volume_of(j.req.shape) * (j.req.nsteps - j.done); }
+
 static bool due(const Job& j) {
     const int every = j.req.measure_every;
     return j.done == j.req.nsteps || (every > 0 && j.done % every == 0);
 }
 
 void lgf_server::run() {
-    std::vector<JobP> active;
+    std::vector<JobP> active;    // admitted and resident
+    std::vector<JobP> stepping;  // the subset that steps in this round
     lgf_batch* batch[2] = {nullptr, nullptr};
     bool dirty = true;
     while (true) {
@@ -156,19 +166,35 @@ void lgf_server::run() {
         if (active.empty()) continue;
         const double tb = now();
         if (dirty) {
+            int pol;
+            int64_t msites;
+            {
+                std::lock_guard<std::mutex> lk(mu);
+                pol = policy;
+                msites = min_sites;
+            }
+            stepping = active;
+            if (pol == LGF_SERVE_SHORTEST_FIRST) {
+                std::stable_sort(stepping.begin(), stepping.end(),
+                                 [](const JobP& a, const JobP& b) { return remaining_work(*a) < remaining_work(*b); });
+                int64_t v = 0;
+                size_t n = 0;
+                while (n < stepping.size() && (n == 0 || v < msites)) v += volume_of(stepping[n++]->req.shape);
+                stepping.resize(n);
+            }
             for (int p = 0; p < 2; ++p) {
                 if (batch[p]) lgf_batch_destroy(batch[p]);
                 batch[p] = nullptr;
                 std::vector<lgf_field*> fs;
-                for (auto& j : active)
+                for (auto& j : stepping)
                     if (j->req.precision == p) fs.push_back(j->field);
                 if (!fs.empty()) lgf_batch_create(ctx, fs.data(), (int)fs.size(), &batch[p]);
             }
             dirty = false;
         }
-        // steps to the nearest event of any active job
+        // steps to the nearest event of any stepping job
         int k = kMaxGroup;
-        for (auto& j : active) {
+        for (auto& j : stepping) {
             int to_end = j->req.nsteps - j->done;
             int every = j->req.measure_every;
             int to_meas = every > 0 ? every - j->done % every : to_end;
@@ -179,7 +205,7 @@ void lgf_server::run() {
         for (int p = 0; p < 2; ++p) {
             if (!batch[p]) continue;
             std::vector<double> eps;
-            for (auto& j : active)
+            for (auto& j : stepping)
                 if (j->req.precision == p) {
                     eps.push_back(j->req.eps);
                     lu += 4.0 * (double)volume_of(j->req.shape) * k;
@@ -187,20 +213,20 @@ void lgf_server::run() {
             int rc = lgf_batch_flow_steps(batch[p], eps.data(), k, 1);
             if (rc != LGF_OK) {
                 set_err(lgf_ctx_last_error(ctx));
-                for (auto& j : active)
+                for (auto& j : stepping)
                     if (j->req.precision == p) j->status = rc;
             }
         }
-        std::vector<JobP> still, done_now;
-        for (auto& j : active) {
+        std::vector<JobP> done_now;
+        for (auto& j : stepping) {
             for (int s = 0; s < k; ++s) j->t += j->req.eps;  // the reference's t += eps per step
             j->done += k;
             if (j->status == LGF_OK && due(*j)) measure_job(this, *j);
-            if (j->done >= j->req.nsteps || j->status != LGF_OK)
-                done_now.push_back(j);
-            else
-                still.push_back(j);
+            if (j->done >= j->req.nsteps || j->status != LGF_OK) done_now.push_back(j);
         }
+        std::vector<JobP> still;
+        for (auto& j : active)
+            if (std::find(done_now.begin(), done_now.end(), j) == done_now.end()) still.push_back(j);
         if (!done_now.empty()) {
             // batches hold the fields: release them before the fields
             for (int p = 0; p < 2; ++p) {
@@ -338,6 +364,17 @@ extern "C" int lgf_server_wait(lgf_server* s, uint64_t id, double timeout_s, lgf
         out->E[i] = j->E[i];
         out->Q[i] = j->Q[i];
     }
+    return LGF_OK;
+}
+
+extern "C" int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites) {
+    if (!s || (policy != LGF_SERVE_ALL && policy != LGF_SERVE_SHORTEST_FIRST) || min_sites < 0) return LGF_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->policy = policy;
+        s->min_sites = min_sites;
+    }
+    s->cv_work.notify_all();
     return LGF_OK;
 }
 

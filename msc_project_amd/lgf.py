@@ -20,6 +20,8 @@ FP64 = 0
 FP32 = 1
 MEASURE_DEFAULT = 0
 MEASURE_FP64 = 1
+SERVE_ALL = 0             # every admitted job steps in every round
+SERVE_SHORTEST_FIRST = 1  # least remaining work first, at least min_sites per round
 # lgf_observable order (include/lgf_flow.h)
 OBSERVABLES = ("E", "Q", "Qabs", "E_improved", "Q_unimproved")
 
@@ -67,6 +69,10 @@ SIGNATURES = {
     "lgf_ctx_set_measure_scratch": ([_vp, ctypes.c_size_t], ctypes.c_int),
     "lgf_measure_observables": ([_vp, ctypes.c_int, _dp, _dp], ctypes.c_int),
     "lgf_batch_measure_observables": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
+    "lgf_batch_measure_start": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lgf_batch_measure_finish": ([_vp, _dp], ctypes.c_int),
+    "lgf_measure_start": ([_vp, ctypes.c_int], ctypes.c_int),
+    "lgf_measure_finish": ([_vp, _dp], ctypes.c_int),
     "lgf_field_footprint": ([_ip, ctypes.c_int, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "lgf_field_create": ([_vp, _ip, ctypes.c_int, ctypes.POINTER(_vp)], ctypes.c_int),
     "lgf_field_destroy": ([_vp], ctypes.c_int),
@@ -92,6 +98,7 @@ SIGNATURES = {
     "lgf_server_submit": ([_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "lgf_server_wait": ([_vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(Result)], ctypes.c_int),
     "lgf_server_now": ([_vp, _dp], ctypes.c_int),
+    "lgf_server_set_policy": ([_vp, ctypes.c_int, ctypes.c_int64], ctypes.c_int),
     "lgf_server_stats": ([_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _dp, _dp],
                          ctypes.c_int),
     "lgf_server_last_error": ([_vp], ctypes.c_char_p),
@@ -291,6 +298,16 @@ class Field:
         out["qraw"] = q
         return out
 
+    def measure_start(self, flags=MEASURE_DEFAULT):
+        """enqueue the clover sweep of the current links; collect with measure_finish"""
+        self.ctx.check(lib().lgf_measure_start(self.h, flags), "measure_start")
+
+    def measure_finish(self):
+        """wait for the outstanding measurement; dict keyed by OBSERVABLES"""
+        obs = np.empty(len(OBSERVABLES))
+        self.ctx.check(lib().lgf_measure_finish(self.h, _p(obs)), "measure_finish")
+        return dict(zip(OBSERVABLES, obs.tolist()))
+
     def diagnostics(self):
         p, mx, mn = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         self.ctx.check(lib().lgf_diagnostics(self.h, ctypes.byref(p), ctypes.byref(mx), ctypes.byref(mn)),
@@ -342,6 +359,17 @@ class Batch:
         self.ctx.check(lib().lgf_batch_measure_observables(self.h, flags, _p(obs)), "batch measure_observables")
         return obs
 
+    def measure_start(self, flags=MEASURE_DEFAULT):
+        """enqueue the clover sweep of the current links; flow steps can be
+        enqueued behind it before the result is collected with measure_finish"""
+        self.ctx.check(lib().lgf_batch_measure_start(self.h, flags), "batch measure_start")
+
+    def measure_finish(self):
+        """wait for the outstanding measurement: array (fields x len(OBSERVABLES))"""
+        obs = np.empty((len(self.fields), len(OBSERVABLES)))
+        self.ctx.check(lib().lgf_batch_measure_finish(self.h, _p(obs)), "batch measure_finish")
+        return obs
+
     def close(self):
         if self.h:
             if self.ctx.h:
@@ -365,6 +393,9 @@ class Server:
             raise LgfError(rc, "lgf_server_create")
         self.h = h
         _live.add(self)
+
+    def set_policy(self, policy, min_sites=1 << 20):
+        self._check(lib().lgf_server_set_policy(self.h, int(policy), int(min_sites)), "set_policy")
 
     def _check(self, rc, what):
         if rc:

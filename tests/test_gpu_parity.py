@@ -193,6 +193,44 @@ def test_measure_in_groups_matches_one_launch(ctx):
             f.close()
 
 
+def test_async_measure_matches_sync(ctx):
+    """a deferred measurement collected after further flow steps were
+    enqueued returns the observables of its own flow time, bitwise equal to a
+    synchronous one, and leaves the flow unchanged"""
+    shapes = [(4, 4, 4, 4), (6, 4, 4, 8)]
+    inputs = [O.generate(s, 6.0, seed=400 + i, start="hot", sweeps=2) for i, s in enumerate(shapes)]
+    for prec in (lgf.FP64, lgf.FP32):
+        ref_fields = [lgf.Field(ctx, s, prec) for s in shapes]
+        fields = [lgf.Field(ctx, s, prec) for s in shapes]
+        for f, g, U in zip(fields, ref_fields, inputs):
+            f.upload(U)
+            g.upload(U)
+        rb, b = lgf.Batch(ctx, ref_fields), lgf.Batch(ctx, fields)
+        for rep in range(3):
+            rb.flow([0.02, 0.01], 2)
+            want = rb.observables(lgf.MEASURE_FP64)
+            rb.flow([0.02, 0.01], 3, use_graph=True)
+            b.flow([0.02, 0.01], 2)
+            b.measure_start(lgf.MEASURE_FP64)
+            with pytest.raises(lgf.LgfError) as e:
+                b.measure_start()
+            assert e.value.code == 7
+            b.flow([0.02, 0.01], 3, use_graph=True)
+            got = b.measure_finish()
+            assert np.array_equal(got, want)
+        for f, g in zip(fields, ref_fields):
+            assert np.array_equal(f.download(), g.download())
+        # the one-field form
+        f = fields[0]
+        f.measure_start()
+        o = f.measure_finish()
+        assert o["E"] == f.observables()["E"]
+        for x in (rb, b):
+            x.close()
+        for x in fields + ref_fields:
+            x.close()
+
+
 def test_graph_replay_matches_direct(ctx):
     shapes = [(4, 4, 4, 4), (6, 4, 4, 8)]
     inputs = [O.generate(s, 6.0, seed=200 + i, start="hot", sweeps=2) for i, s in enumerate(shapes)]

@@ -1,6 +1,6 @@
 """Serving loop: jobs that arrive while others flow are admitted by device
 footprint, run batched, and each job's results are bitwise those of the same
-job flowed alone."""
+job flowed alone, under either scheduling policy."""
 import time
 
 import numpy as np
@@ -32,11 +32,14 @@ def alone(ctx, shape, eps, nsteps, every, prec, links=None, seed=None):
     return np.array(t), np.array(E), np.array(Q)
 
 
-def test_server_results_same_alone_and_in_mix():
+@pytest.mark.parametrize("policy,min_sites", [(lgf.SERVE_SHORTEST_FIRST, 1 << 20), (lgf.SERVE_ALL, 0),
+                                              (lgf.SERVE_SHORTEST_FIRST, 300)])
+def test_server_results_same_alone_and_in_mix(policy, min_sites):
     jobs = [((4, 4, 4, 4), 0.02, 23, 5, lgf.FP32, 11), ((8, 4, 4, 6), 0.01, 17, 4, lgf.FP64, 12),
             ((6, 6, 4, 4), 0.03, 30, 10, lgf.FP32, 13), ((4, 4, 8, 8), 0.01, 9, 3, lgf.FP64, 14),
             ((4, 4, 4, 4), 0.02, 40, 7, lgf.FP32, 15)]
     srv = lgf.Server(0)
+    srv.set_policy(policy, min_sites)  # 300 sites: only the shortest one or two jobs step at a time
     ids = []
     for k, (shape, eps, n, every, prec, seed) in enumerate(jobs):
         ids.append(srv.submit(shape, eps, n, every, precision=prec, seed=seed))
