@@ -83,7 +83,7 @@ def thermalised(lgf, ctx, shape, beta, seed, sweeps):
 
 def run_job(lgf, ctx, src, prec, steps, warmup, ev):
     """One flow job on a copy of `src`: returns timings and the measured series."""
-    variant, tune_ms = ctx.autotune(SHAPE, prec, 6)
+    variant, tune_ms = ctx.autotune(SHAPE, prec, 10)
     flags = lgf.MEASURE_DEFAULT  # clover arithmetic in the path's precision, FP64 reductions
     # warm-up on a scratch copy (captures graphs, touches every kernel)
     w = lgf.Field(ctx, SHAPE, prec)

@@ -110,8 +110,9 @@ int lgf_ctx_get_tile_order(lgf_ctx* ctx, int* block);
 int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes);
 /* choose the tile order (fastest of 0, 2, 4, 8 with variant 0), then time
  * every stage variant with it for nsteps RK3 steps on a seeded lattice of
- * `shape`, and select the fastest for this context; ms_per_step (optional)
- * gets one entry per variant */
+ * `shape` (each candidate three times, its fastest run counting), and select
+ * the fastest for this context; ms_per_step (optional) gets one entry per
+ * variant */
 int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best, double* ms_per_step);
 
 /* ---- one lattice ------------------------------------------------------ */

@@ -782,6 +782,19 @@ extern "C" int lgf_measure_finish(lgf_field* f, double* obs) {
 // shape (nsteps RK3 steps each, after one warm-up step, HIP events on the
 // context's stream) and makes the fastest the context's variant for that
 // precision.  ms_per_step (optional) receives one time per variant.
+// each candidate is timed kAutotuneReps times and its fastest run counts
+constexpr int kAutotuneReps = 3;
+
+static int time_steps(lgf_ctx* ctx, lgf_batch* b, double eps, int nsteps, hipEvent_t e0, hipEvent_t e1, float* ms) {
+    if (hipEventRecord(e0, ctx->stream) != hipSuccess) return lgf_fail(ctx, LGF_E_HIP, "event record");
+    int rc = lgf_batch_flow_steps(b, &eps, nsteps, 0);
+    if (rc) return rc;
+    if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+        return lgf_fail(ctx, LGF_E_HIP, "event timing");
+    return LGF_OK;
+}
+
 extern "C" int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best,
                             double* ms_per_step) {
     if (!ctx || nsteps <= 0 || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
@@ -809,12 +822,12 @@ extern "C" int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int
             lgf_batch* bo = nullptr;
             rc = lgf_batch_create(ctx, &f, 1, &bo);
             if (!rc) rc = lgf_batch_flow_steps(bo, &eps, 1, 0);
-            if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = LGF_E_HIP;
-            if (!rc) rc = lgf_batch_flow_steps(bo, &eps, nsteps, 0);
-            if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LGF_E_HIP;
             float ms = 0.f;
-            if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-                rc = lgf_fail(ctx, LGF_E_HIP, "event timing");
+            for (int r = 0; r < kAutotuneReps && !rc; ++r) {
+                float m = 0.f;
+                rc = time_steps(ctx, bo, eps, nsteps, e0, e1, &m);
+                if (!rc && (r == 0 || m < ms)) ms = m;
+            }
             if (bo) lgf_batch_destroy(bo);
             if (!rc && ms < best_order_ms) {
                 best_order_ms = ms;
@@ -830,12 +843,12 @@ extern "C" int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int
     for (int v = 0; v < nv && !rc; ++v) {
         ctx->stage_variant[precision] = v;
         rc = lgf_batch_flow_steps(b, &eps, 1, 0);
-        if (!rc && hipEventRecord(e0, ctx->stream) != hipSuccess) rc = LGF_E_HIP;
-        if (!rc) rc = lgf_batch_flow_steps(b, &eps, nsteps, 0);
-        if (!rc && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = LGF_E_HIP;
         float ms = 0.f;
-        if (!rc && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
-            rc = lgf_fail(ctx, LGF_E_HIP, "event timing");
+        for (int r = 0; r < kAutotuneReps && !rc; ++r) {
+            float m = 0.f;
+            rc = time_steps(ctx, b, eps, nsteps, e0, e1, &m);
+            if (!rc && (r == 0 || m < ms)) ms = m;
+        }
         if (rc) break;
         const double per = (double)ms / nsteps;
         if (ms_per_step) ms_per_step[v] = per;

@@ -21,6 +21,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "expm.hpp"
 #include "kernels.hpp"
 #include "lattice.hpp"
@@ -33,17 +35,25 @@ namespace lgf {
 // ---------------------------------------------------------------------------
 // RK3 stage
 // ---------------------------------------------------------------------------
-// BY = directions per workgroup: 4 (block 64x4, mu = threadIdx.y: the four
-// links of a site share their neighbours in L1/L2) or 1 (block 64, mu =
-// blockIdx.y).  WPE = minimum waves per SIMD the register budget must allow.
-template <typename S, typename C, int STAGE, int EXPMODE, int WPE, int BY>
+// Launch shapes.  BY = waves per workgroup along y, MPT = directions each
+// thread updates, one after the other:
+//   BY 4, MPT 1: block 64x4, mu = threadIdx.y (the four links of a site share
+//                their neighbours in L1/L2);
+//   BY 1, MPT 1: block 64, mu = blockIdx.y;
+//   BY 2, MPT 2 and BY 1, MPT 4: a thread updates 2 (4) directions of its
+//                site in turn, so an XCD keeps 2 (4) times fewer sites in
+//                flight and their neighbourhood fits its L2 better.
+// WPE = minimum waves per SIMD the register budget must allow.
+template <typename S, typename C, int STAGE, int EXPMODE, int WPE, int BY, int MPT>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage(
     const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int ping) {
+    static_assert(MPT == 1 || BY * MPT == 4, "a workgroup covers the four directions of its sites");
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int2 tm = tiles[t];
     const FieldDesc& d = fds[tm.x];
     const int s = tm.y * kTileSites + threadIdx.x;
-    const int mu = BY == 4 ? __builtin_amdgcn_readfirstlane(threadIdx.y) : (int)blockIdx.y;
+    const int mu0 = MPT > 1 ? __builtin_amdgcn_readfirstlane(threadIdx.y) * MPT
+                            : (BY == 4 ? __builtin_amdgcn_readfirstlane(threadIdx.y) : (int)blockIdx.y);
     if (s >= d.vol) return;
     const int cur = ping ^ d.flip;
     const S* __restrict__ U = (const S*)d.links[cur];
@@ -51,52 +61,72 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8)
     S* __restrict__ Xa = (S*)d.acc;
     const int64_t st = d.stride;
     const Geo g = make_geo(d, s);
-    const M3<C> A = staple<S, C>(U, st, g, mu);
-    const M3<C> Umu = load_link<S, C>(U, st, mu, s);
-    const M3<C> Z = ta_scaled(mul_nn(Umu, A), C(-0.5));
-
     const C eps = (C)d.eps;
-    M3<C> E;
-    if (STAGE == 0) {
-        store_ta<S, C>(Xa, st, mu, s, Z);
-#pragma unroll
-        for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
-    } else if (STAGE == 1) {
-        M3<C> X0 = load_ta<S, C>(Xa, st, mu, s);
-        M3<C> X1;
-#pragma unroll
-        for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
-        store_ta<S, C>(Xa, st, mu, s, X1);
-#pragma unroll
-        for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
-    } else {
-        M3<C> X1 = load_ta<S, C>(Xa, st, mu, s);
-#pragma unroll
-        for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
-    }
+#pragma unroll 1
+    for (int m = 0; m < MPT; ++m) {
+        const int mu = mu0 + m;
+        const M3<C> A = staple<S, C>(U, st, g, mu);
+        const M3<C> Umu = load_link<S, C>(U, st, mu, s);
+        const M3<C> Z = ta_scaled(mul_nn(Umu, A), C(-0.5));
 
-    M3<C> ex;
-    if constexpr (EXPMODE == 0) {
-        ex = exp_ah_cayley(E);  // C == double
-    } else {
-        ex = exp_ah_series<C>(E);
-    }
-    M3<C> Unew = mul_nn(ex, Umu);
-    if constexpr (EXPMODE == 1 && STAGE == 2) {
-        // once per RK3 step: project back onto SU(3) in double precision
-        M3<double> Ud;
+        M3<C> E;
+        if (STAGE == 0) {
+            store_ta<S, C>(Xa, st, mu, s, Z);
 #pragma unroll
-        for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
-        reunitarize(Ud);
-        store_link<S, double>(Un, st, mu, s, Ud);
-    } else {
-        store_link<S, C>(Un, st, mu, s, Unew);
+            for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
+        } else if (STAGE == 1) {
+            const M3<C> X0 = load_ta<S, C>(Xa, st, mu, s);
+            M3<C> X1;
+#pragma unroll
+            for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
+            store_ta<S, C>(Xa, st, mu, s, X1);
+#pragma unroll
+            for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
+        } else {
+            const M3<C> X1 = load_ta<S, C>(Xa, st, mu, s);
+#pragma unroll
+            for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
+        }
+
+        M3<C> ex;
+        if constexpr (EXPMODE == 0) {
+            ex = exp_ah_cayley(E);  // C == double
+        } else {
+            ex = exp_ah_series<C>(E);
+        }
+        M3<C> Unew = mul_nn(ex, Umu);
+        if constexpr (EXPMODE == 1 && STAGE == 2) {
+            // once per RK3 step: project back onto SU(3) in double precision
+            M3<double> Ud;
+#pragma unroll
+            for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
+            reunitarize(Ud);
+            store_link<S, double>(Un, st, mu, s, Ud);
+        } else {
+            store_link<S, C>(Un, st, mu, s, Unew);
+        }
     }
 }
 
 // ---------------------------------------------------------------------------
 // clovers, E, Q, q(x)
 // ---------------------------------------------------------------------------
+// Compile-time knobs of the clover kernel:
+//   LGF_CLOVER_FENCE    1: a leaf's loads are not hoisted into the previous leaf
+//   LGF_CLOVER_WPE      minimum waves per SIMD of the register budget
+//   LGF_CLOVER_LDS_REAL 1: per-plane field strengths meet in LDS in the
+//                       arithmetic precision (float for FP32 clovers), which
+//                       halves LDS use and lets more workgroups share a CU
+#ifndef LGF_CLOVER_FENCE
+#define LGF_CLOVER_FENCE 0
+#endif
+#ifndef LGF_CLOVER_WPE
+#define LGF_CLOVER_WPE 2
+#endif
+#ifndef LGF_CLOVER_LDS_REAL
+#define LGF_CLOVER_LDS_REAL 0
+#endif
+
 // Line products.  L^d_l(y) = U_d(y) U_d(y+d) ... U_d(y+(l-1)d), the product
 // of l links along a straight line.  k_lines stores L_2 and L_3 of every
 // site and direction of the fields being measured (precision C, the vector
@@ -159,7 +189,9 @@ LGF_DEV void scale(M3<C>& a, C w) {
 // diagonal quadrants: it = 0 is (+nu,+mu), it = 1 is (-nu,-mu)
 template <int M, int N, typename S, typename C>
 LGF_DEV void diag_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
+#if LGF_CLOVER_FENCE
     asm volatile("" ::: "memory");  // keep the next leaf's loads from being hoisted into this one
+#endif
     const int mb = o.m[3 - M], nb = o.n[3 - N], mf = o.m[3 + M], nf = o.n[3 + N];
     const int sc = it ? s + nb + mb : s;
     const int scn = it ? s + mb : s + nf;  // c + n nu
@@ -175,7 +207,9 @@ LGF_DEV void diag_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const Plane
 // is (-mu,+nu) with (alpha, beta) = (nu, mu), entering as -W^dag
 template <int L1, int L2, typename S, typename C>
 LGF_DEV void side_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
+#if LGF_CLOVER_FENCE
     asm volatile("" ::: "memory");
+#endif
     const int a = it ? nu : mu, b = it ? mu : nu;
     const int m2 = o.m[3 - L2], n2 = o.n[3 - L2], m1 = o.m[3 + L1], n1 = o.n[3 + L1];
     const int zb = it ? m2 : n2;  // z = x - l2 beta
@@ -205,11 +239,11 @@ LGF_DEV M3<C> project_F(const M3<C>& Cl) {
 // 588-597), where each F is the projection of its clover average: 1/4 of
 // four square leaves, 1/8 of eight rectangle leaves.  The projection is
 // linear, so it is applied once to the weighted sum.  Returns Re Tr(F11 F11).
-template <typename C>
-LGF_DEV void herm_pack(const M3<C>& F, double* h, int step);
+template <typename C, typename T>
+LGF_DEV void herm_pack(const M3<C>& F, T* h, int step);
 
-template <typename S, typename C>
-LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fimp, double* f11) {
+template <typename S, typename C, typename T>
+LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fimp, T* f11) {
     PlaneOff o;
 #pragma unroll
     for (int k = -3; k <= 3; ++k) {
@@ -254,22 +288,26 @@ LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fim
 
 // Re Tr(A B) for Hermitian A, B given as 9 reals each:
 // (A00, A11, A22, Re A01, Im A01, Re A02, Im A02, Re A12, Im A12)
-LGF_DEV double herm_dot(const double* a, const double* b) {
-    return a[0] * b[0] + a[1] * b[1] + a[2] * b[2] +
-           2.0 * (a[3] * b[3] + a[4] * b[4] + a[5] * b[5] + a[6] * b[6] + a[7] * b[7] + a[8] * b[8]);
+template <typename T>
+LGF_DEV double herm_dot(const T* a, const T* b) {
+    double x[9], y[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) x[k] = (double)a[k], y[k] = (double)b[k];
+    return x[0] * y[0] + x[1] * y[1] + x[2] * y[2] +
+           2.0 * (x[3] * y[3] + x[4] * y[4] + x[5] * y[5] + x[6] * y[6] + x[7] * y[7] + x[8] * y[8]);
 }
 
-template <typename C>
-LGF_DEV void herm_pack(const M3<C>& F, double* h, int step) {
-    h[0 * step] = (double)F.v[0];
-    h[1 * step] = (double)F.v[8];
-    h[2 * step] = (double)F.v[16];
-    h[3 * step] = (double)F.v[2];
-    h[4 * step] = (double)F.v[3];
-    h[5 * step] = (double)F.v[4];
-    h[6 * step] = (double)F.v[5];
-    h[7 * step] = (double)F.v[10];
-    h[8 * step] = (double)F.v[11];
+template <typename C, typename T>
+LGF_DEV void herm_pack(const M3<C>& F, T* h, int step) {
+    h[0 * step] = (T)F.v[0];
+    h[1 * step] = (T)F.v[8];
+    h[2 * step] = (T)F.v[16];
+    h[3 * step] = (T)F.v[2];
+    h[4 * step] = (T)F.v[3];
+    h[5 * step] = (T)F.v[4];
+    h[6 * step] = (T)F.v[5];
+    h[7 * step] = (T)F.v[10];
+    h[8 * step] = (T)F.v[11];
 }
 
 // L_2 and L_3 of one (site, direction) of a chunk of the batch's tiles
@@ -306,11 +344,13 @@ __global__ __launch_bounds__(256) void k_lines(const FieldDesc* __restrict__ fds
 // requested, q(x).  Reads the line products of the same chunk from k_lines.
 
 template <typename S, typename C>
-__global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) void k_clover(
+__global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(LGF_CLOVER_WPE, 8))) void k_clover(
     const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int t0, int ping, const C* __restrict__ lines,
     int64_t G, double* __restrict__ partials) {
-    __shared__ double sF[6][9][kTileSites];
-    __shared__ double sF11[6][9][kTileSites];
+    // F values are computed in C; keeping them as C in LDS loses nothing
+    using T = typename std::conditional<LGF_CLOVER_LDS_REAL != 0, C, double>::type;
+    __shared__ T sF[6][9][kTileSites];
+    __shared__ T sF11[6][9][kTileSites];
     __shared__ double sE[6][kTileSites];
     __shared__ double sEi[6][kTileSites];
     const int tix = xcd_remap(blockIdx.x, gridDim.x);
@@ -333,18 +373,19 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
         const int mu = (p & 1) ? (p == 1 ? 0 : (p == 3 ? 2 : 1)) : 3;
         const int nu = (p & 1) ? (p == 1 ? 1 : (p == 3 ? 0 : 2)) : (p == 0 ? 2 : (p == 2 ? 1 : 0));
         M3<C> Fi;
-        e = (double)plane_F<S, C>(L, g, mu, nu, Fi, &sF11[p][0][threadIdx.x]);
+        e = (double)plane_F<S, C, T>(L, g, mu, nu, Fi, &sF11[p][0][threadIdx.x]);
         ei = (double)retr_mul(Fi, Fi);
         herm_pack(Fi, &sF[p][0][threadIdx.x], kTileSites);
     } else {
 #pragma unroll
-        for (int i = 0; i < 9; ++i) sF[p][i][threadIdx.x] = sF11[p][i][threadIdx.x] = 0.0;
+        for (int i = 0; i < 9; ++i) sF[p][i][threadIdx.x] = sF11[p][i][threadIdx.x] = T(0);
     }
     sE[p][threadIdx.x] = e;
     sEi[p][threadIdx.x] = ei;
     __syncthreads();
     if (p == 0) {
-        double a[9], b[9], qs = 0.0, q11 = 0.0, es = 0.0, eis = 0.0;
+        T a[9], b[9];
+        double qs = 0.0, q11 = 0.0, es = 0.0, eis = 0.0;
 #pragma unroll 1
         for (int k = 0; k < 3; ++k) {
 #pragma unroll
@@ -394,19 +435,28 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(2, 8))) voi
 // sum ei, sum q11}.
 __global__ __launch_bounds__(256) void k_reduce(const FieldDesc* __restrict__ fds, const double* __restrict__ partials,
                                                 int pstride, int ncomp, double* __restrict__ out) {
-    __shared__ double red[256];
+    __shared__ double red[8][4];
     const FieldDesc& d = fds[blockIdx.x];
-    for (int c = 0; c < ncomp; ++c) {
-        double acc = 0.0;
-        for (int t = threadIdx.x; t < d.ntiles; t += 256) acc += partials[pstride * (d.tile0 + t) + c];
-        red[threadIdx.x] = acc;
-        __syncthreads();
-        for (int off = 128; off > 0; off >>= 1) {
-            if ((int)threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[kOutStride * blockIdx.x + c] = red[0];
-        __syncthreads();
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    double acc[8];
+#pragma unroll
+    for (int c = 0; c < 8; ++c) acc[c] = 0.0;
+    for (int t = threadIdx.x; t < d.ntiles; t += 256) {
+        const double* rec = partials + (int64_t)pstride * (d.tile0 + t);
+#pragma unroll
+        for (int c = 0; c < 8; ++c)
+            if (c < ncomp) acc[c] += rec[c];
+    }
+#pragma unroll
+    for (int c = 0; c < 8; ++c) {
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) acc[c] += __shfl_down(acc[c], off, 64);
+        if (lane == 0) red[c][w] = acc[c];
+    }
+    __syncthreads();
+    if ((int)threadIdx.x < ncomp) {
+        const int c = threadIdx.x;
+        out[kOutStride * blockIdx.x + c] = (red[c][0] + red[c][1]) + (red[c][2] + red[c][3]);
     }
 }
 
@@ -544,22 +594,22 @@ __global__ __launch_bounds__(256) void k_cold(S* __restrict__ dst, int64_t strid
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-template <typename S, typename C, int EXPMODE, int WPE, int BY>
+template <typename S, typename C, int EXPMODE, int WPE, int BY, int MPT = 1>
 static hipError_t stage_launch(int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
                                hipStream_t stream) {
-    dim3 grid(ntiles, 4 / BY), block(kTileSites, BY);
+    dim3 grid(ntiles, MPT > 1 ? 1 : 4 / BY), block(kTileSites, BY);
     if (stage == 0)
-        hipLaunchKernelGGL((k_rk3_stage<S, C, 0, EXPMODE, WPE, BY>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 0, EXPMODE, WPE, BY, MPT>), grid, block, 0, stream, fds, tiles, ping);
     else if (stage == 1)
-        hipLaunchKernelGGL((k_rk3_stage<S, C, 1, EXPMODE, WPE, BY>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 1, EXPMODE, WPE, BY, MPT>), grid, block, 0, stream, fds, tiles, ping);
     else
-        hipLaunchKernelGGL((k_rk3_stage<S, C, 2, EXPMODE, WPE, BY>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage<S, C, 2, EXPMODE, WPE, BY, MPT>), grid, block, 0, stream, fds, tiles, ping);
     return hipGetLastError();
 }
 
-// Stage-kernel variants: (register budget as minimum waves per SIMD, block
+// Stage-kernel variants: (register budget as minimum waves per SIMD, launch
 // shape).  Variant 0 is the default; lgf_autotune picks the fastest.
-int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 6; }
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 7 : 8; }
 
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream) {
@@ -569,6 +619,8 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
             case 2: return stage_launch<double, double, 0, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
             case 3: return stage_launch<double, double, 0, 3, 1>(stage, fds, tiles, ntiles, ping, stream);
             case 4: return stage_launch<double, double, 0, 2, 1>(stage, fds, tiles, ntiles, ping, stream);
+            case 5: return stage_launch<double, double, 0, 2, 2, 2>(stage, fds, tiles, ntiles, ping, stream);
+            case 6: return stage_launch<double, double, 0, 2, 1, 4>(stage, fds, tiles, ntiles, ping, stream);
             default: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
         }
     }
@@ -578,6 +630,8 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
         case 3: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
         case 4: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
         case 5: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
+        case 6: return stage_launch<float, float, 1, 5, 2, 2>(stage, fds, tiles, ntiles, ping, stream);
+        case 7: return stage_launch<float, float, 1, 5, 1, 4>(stage, fds, tiles, ntiles, ping, stream);
         default: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
     }
 }

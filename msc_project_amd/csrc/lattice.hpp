@@ -183,9 +183,8 @@ LGF_DEV void store_link(S* __restrict__ base, int64_t stride, int mu, int s, con
 //   FP32: two float4 planes (a0..a3, a4..a7) and one float plane (a8);
 //   FP64: four double2 planes and one double plane.
 template <typename S, typename C>
-LGF_DEV M3<C> load_ta(const S* __restrict__ base, int64_t stride, int mu, int s) {
+LGF_DEV void load_ta_raw(const S* __restrict__ base, int64_t stride, int mu, int s, C* a) {
     const S* p = base + (int64_t)mu * 9 * stride;
-    C a[9];
     if constexpr (sizeof(S) == 4) {
         const vf4 v0 = ld<vf4>(p, (uint32_t)s * 16u), v1 = ld<vf4>(p + 4 * stride, (uint32_t)s * 16u);
         a[0] = v0.x, a[1] = v0.y, a[2] = v0.z, a[3] = v0.w;
@@ -200,12 +199,23 @@ LGF_DEV M3<C> load_ta(const S* __restrict__ base, int64_t stride, int mu, int s)
         }
         a[8] = (C)ld<double>(p + 8 * stride, (uint32_t)s * 8u);
     }
+}
+
+template <typename C>
+LGF_DEV M3<C> expand_ta(const C* a) {
     M3<C> m;
     m.v[0] = C(0), m.v[1] = a[0], m.v[8] = C(0), m.v[9] = a[1], m.v[16] = C(0), m.v[17] = a[2];
     m.v[2] = a[3], m.v[3] = a[4], m.v[6] = -a[3], m.v[7] = a[4];      // (0,1), (1,0)
     m.v[4] = a[5], m.v[5] = a[6], m.v[12] = -a[5], m.v[13] = a[6];    // (0,2), (2,0)
     m.v[10] = a[7], m.v[11] = a[8], m.v[14] = -a[7], m.v[15] = a[8];  // (1,2), (2,1)
     return m;
+}
+
+template <typename S, typename C>
+LGF_DEV M3<C> load_ta(const S* __restrict__ base, int64_t stride, int mu, int s) {
+    C a[9];
+    load_ta_raw<S, C>(base, stride, mu, s, a);
+    return expand_ta(a);
 }
 
 template <typename S, typename C>
