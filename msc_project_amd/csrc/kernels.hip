@@ -35,6 +35,54 @@ namespace lgf {
 // ---------------------------------------------------------------------------
 // RK3 stage
 // ---------------------------------------------------------------------------
+// One link update of one RK3 stage, the direction MU a compile-time
+// constant (the kernel switches on the wave-uniform direction once).
+template <typename S, typename C, int STAGE, int EXPMODE, int MU>
+LGF_DEV void stage_link(const S* __restrict__ U, S* __restrict__ Un, S* __restrict__ Xa, int64_t st, const Geo& g,
+                        C eps) {
+    const int s = g.s;
+    const M3<C> A = staple_ct<S, C, MU>(U, st, g);
+    const M3<C> Umu = load_link<S, C>(U, st, MU, s);
+    const M3<C> Z = ta_scaled(mul_nn(Umu, A), C(-0.5));
+
+    M3<C> E;
+    if (STAGE == 0) {
+        store_ta<S, C>(Xa, st, MU, s, Z);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
+    } else if (STAGE == 1) {
+        const M3<C> X0 = load_ta<S, C>(Xa, st, MU, s);
+        M3<C> X1;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
+        store_ta<S, C>(Xa, st, MU, s, X1);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
+    } else {
+        const M3<C> X1 = load_ta<S, C>(Xa, st, MU, s);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
+    }
+
+    M3<C> ex;
+    if constexpr (EXPMODE == 0) {
+        ex = exp_ah_cayley(E);  // C == double
+    } else {
+        ex = exp_ah_series<C>(E);
+    }
+    M3<C> Unew = mul_nn(ex, Umu);
+    if constexpr (EXPMODE == 1 && STAGE == 2) {
+        // once per RK3 step: project back onto SU(3) in double precision
+        M3<double> Ud;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
+        reunitarize(Ud);
+        store_link<S, double>(Un, st, MU, s, Ud);
+    } else {
+        store_link<S, C>(Un, st, MU, s, Unew);
+    }
+}
+
 // Launch shapes.  BY = waves per workgroup along y, MPT = directions each
 // thread updates, one after the other:
 //   BY 4, MPT 1: block 64x4, mu = threadIdx.y (the four links of a site share
@@ -42,7 +90,7 @@ namespace lgf {
 //   BY 1, MPT 1: block 64, mu = blockIdx.y;
 //   BY 2, MPT 2 and BY 1, MPT 4: a thread updates 2 (4) directions of its
 //                site in turn, so an XCD keeps 2 (4) times fewer sites in
-//                flight and their neighbourhood fits its L2 better.
+//                flight.
 // WPE = minimum waves per SIMD the register budget must allow.
 template <typename S, typename C, int STAGE, int EXPMODE, int WPE, int BY, int MPT>
 __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage(
@@ -64,46 +112,11 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8)
     const C eps = (C)d.eps;
 #pragma unroll 1
     for (int m = 0; m < MPT; ++m) {
-        const int mu = mu0 + m;
-        const M3<C> A = staple<S, C>(U, st, g, mu);
-        const M3<C> Umu = load_link<S, C>(U, st, mu, s);
-        const M3<C> Z = ta_scaled(mul_nn(Umu, A), C(-0.5));
-
-        M3<C> E;
-        if (STAGE == 0) {
-            store_ta<S, C>(Xa, st, mu, s, Z);
-#pragma unroll
-            for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
-        } else if (STAGE == 1) {
-            const M3<C> X0 = load_ta<S, C>(Xa, st, mu, s);
-            M3<C> X1;
-#pragma unroll
-            for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
-            store_ta<S, C>(Xa, st, mu, s, X1);
-#pragma unroll
-            for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
-        } else {
-            const M3<C> X1 = load_ta<S, C>(Xa, st, mu, s);
-#pragma unroll
-            for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
-        }
-
-        M3<C> ex;
-        if constexpr (EXPMODE == 0) {
-            ex = exp_ah_cayley(E);  // C == double
-        } else {
-            ex = exp_ah_series<C>(E);
-        }
-        M3<C> Unew = mul_nn(ex, Umu);
-        if constexpr (EXPMODE == 1 && STAGE == 2) {
-            // once per RK3 step: project back onto SU(3) in double precision
-            M3<double> Ud;
-#pragma unroll
-            for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
-            reunitarize(Ud);
-            store_link<S, double>(Un, st, mu, s, Ud);
-        } else {
-            store_link<S, C>(Un, st, mu, s, Unew);
+        switch (mu0 + m) {
+            case 0: stage_link<S, C, STAGE, EXPMODE, 0>(U, Un, Xa, st, g, eps); break;
+            case 1: stage_link<S, C, STAGE, EXPMODE, 1>(U, Un, Xa, st, g, eps); break;
+            case 2: stage_link<S, C, STAGE, EXPMODE, 2>(U, Un, Xa, st, g, eps); break;
+            default: stage_link<S, C, STAGE, EXPMODE, 3>(U, Un, Xa, st, g, eps); break;
         }
     }
 }
@@ -609,7 +622,7 @@ static hipError_t stage_launch(int stage, const FieldDesc* fds, const int2* tile
 
 // Stage-kernel variants: (register budget as minimum waves per SIMD, launch
 // shape).  Variant 0 is the default; lgf_autotune picks the fastest.
-int num_stage_variants(int prec) { return prec == kPrecFP64 ? 7 : 8; }
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 6; }
 
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream) {
@@ -619,8 +632,6 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
             case 2: return stage_launch<double, double, 0, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
             case 3: return stage_launch<double, double, 0, 3, 1>(stage, fds, tiles, ntiles, ping, stream);
             case 4: return stage_launch<double, double, 0, 2, 1>(stage, fds, tiles, ntiles, ping, stream);
-            case 5: return stage_launch<double, double, 0, 2, 2, 2>(stage, fds, tiles, ntiles, ping, stream);
-            case 6: return stage_launch<double, double, 0, 2, 1, 4>(stage, fds, tiles, ntiles, ping, stream);
             default: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
         }
     }
@@ -630,8 +641,6 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
         case 3: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
         case 4: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
         case 5: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
-        case 6: return stage_launch<float, float, 1, 5, 2, 2>(stage, fds, tiles, ntiles, ping, stream);
-        case 7: return stage_launch<float, float, 1, 5, 1, 4>(stage, fds, tiles, ntiles, ping, stream);
         default: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
     }
 }

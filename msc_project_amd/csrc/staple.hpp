@@ -37,4 +37,42 @@ LGF_DEV M3<C> staple(const S* __restrict__ U, int64_t st, const Geo& g, int mu) 
     return A;
 }
 
+// The same sum with the direction a compile-time constant: every neighbour
+// offset and plane base is then a constant expression of the site's
+// coordinates, with no selects on a run-time direction.  The terms are added
+// in the order of staple() (nu ascending, upper before lower).
+template <typename S, typename C, int MU, int NU>
+LGF_DEV void staple_term(M3<C>& A, const S* __restrict__ U, int64_t st, const Geo& g, int fmu) {
+    const int s = g.s;
+    const int fnu = nb_off(g, NU, +1), bnu = nb_off(g, NU, -1);
+    {
+        M3<C> a = load_link<S, C>(U, st, NU, s);
+        M3<C> b = load_link<S, C>(U, st, MU, s + fnu);
+        M3<C> tt = mul_nn(a, b);
+        M3<C> c = load_link<S, C>(U, st, NU, s + fmu);
+        mul_nd_acc(A, c, tt);
+    }
+    {
+        M3<C> a = load_link<S, C>(U, st, MU, s + bnu);
+        M3<C> b = load_link<S, C>(U, st, NU, s + bnu + fmu);
+        M3<C> tt = mul_nn(a, b);
+        M3<C> c = load_link<S, C>(U, st, NU, s + bnu);
+        mul_dn_acc(A, tt, c);
+    }
+}
+
+template <typename S, typename C, int MU>
+LGF_DEV M3<C> staple_ct(const S* __restrict__ U, int64_t st, const Geo& g) {
+    constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
+    const int fmu = nb_off(g, MU, +1);
+    M3<C> A;
+    set_zero(A);
+    staple_term<S, C, MU, N0>(A, U, st, g, fmu);
+    asm volatile("" ::: "memory");  // one term's loads at a time: bounds the live registers
+    staple_term<S, C, MU, N1>(A, U, st, g, fmu);
+    asm volatile("" ::: "memory");
+    staple_term<S, C, MU, N2>(A, U, st, g, fmu);
+    return A;
+}
+
 }  // namespace lgf
