@@ -621,27 +621,28 @@ static hipError_t stage_launch(int stage, const FieldDesc* fds, const int2* tile
 }
 
 // Stage-kernel variants: (register budget as minimum waves per SIMD, launch
-// shape).  Variant 0 is the default; lgf_autotune picks the fastest.
+// shape).  Variant 0 is the default (a budget the kernel fits without
+// spilling); lgf_autotune picks the fastest.
 int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 6; }
 
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream) {
     if (prec == kPrecFP64) {
         switch (variant) {
-            case 1: return stage_launch<double, double, 0, 2, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 1: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
             case 2: return stage_launch<double, double, 0, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
             case 3: return stage_launch<double, double, 0, 3, 1>(stage, fds, tiles, ntiles, ping, stream);
             case 4: return stage_launch<double, double, 0, 2, 1>(stage, fds, tiles, ntiles, ping, stream);
-            default: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+            default: return stage_launch<double, double, 0, 2, 4>(stage, fds, tiles, ntiles, ping, stream);
         }
     }
     switch (variant) {
-        case 1: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 2: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 3: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 1: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 2: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 3: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
         case 4: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
         case 5: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
-        default: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
+        default: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
     }
 }
 
