@@ -90,6 +90,53 @@ LGF_DEV M3<double> exp_ah_cayley(const M3<double>& X) {
     return o;
 }
 
+// Q^2 and Re Tr(Q^3) for Hermitian Q (the exponent's -iX, X anti-Hermitian
+// with exactly mirrored lower triangle): Q^2 is Hermitian with a real
+// diagonal, so only its diagonal and upper triangle are computed (54 FMAs
+// instead of 108) and the lower triangle is their conjugate.
+template <typename R>
+LGF_DEV void herm_square(const M3<R>& Q, M3<R>& Q2, R& tr3) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r) {
+        R d = R(0);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            d += Q.v[2 * (3 * r + k)] * Q.v[2 * (3 * r + k)];
+            d += Q.v[2 * (3 * r + k) + 1] * Q.v[2 * (3 * r + k) + 1];
+        }
+        Q2.v[8 * r] = d;
+        Q2.v[8 * r + 1] = R(0);
+    }
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = r + 1; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                LGF_CMAC(x, y, Q.v[2 * (3 * r + k)], Q.v[2 * (3 * r + k) + 1], Q.v[2 * (3 * k + c)],
+                         Q.v[2 * (3 * k + c) + 1])
+            }
+            Q2.v[2 * (3 * r + c)] = x;
+            Q2.v[2 * (3 * r + c) + 1] = y;
+            Q2.v[2 * (3 * c + r)] = x;
+            Q2.v[2 * (3 * c + r) + 1] = -y;
+        }
+    // Tr(Q Q^2) = sum_r Q_rr Q2_rr + 2 sum_{r<c} Re(Q_rc conj(Q2_rc))
+    R t = Q.v[0] * Q2.v[0];
+    t += Q.v[8] * Q2.v[8];
+    t += Q.v[16] * Q2.v[16];
+    R u = R(0);
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = r + 1; c < 3; ++c) {
+            u += Q.v[2 * (3 * r + c)] * Q2.v[2 * (3 * r + c)];
+            u += Q.v[2 * (3 * r + c) + 1] * Q2.v[2 * (3 * r + c) + 1];
+        }
+    tr3 = t + R(2) * u;
+}
+
 // Power series through the Cayley-Hamilton recurrence, real type R.
 // Truncated after Q^(NTERMS-1): with c1 <= C1MAX every eigenvalue satisfies
 // |lambda| <= sqrt(4 c1 / 3) and the first omitted term is below
@@ -106,7 +153,9 @@ LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
         Q.v[2 * i] = X.v[2 * i + 1];
         Q.v[2 * i + 1] = -X.v[2 * i];
     }
-    M3<R> Q2 = mul_nn(Q, Q);
+    M3<R> Q2;
+    R tr3;
+    herm_square(Q, Q2, tr3);
     R c1 = retr(Q2) * R(0.5);
     int k = 0;
     if (c1 > R(C1MAX)) {  // scaling: c1 ~ |Q|^2 shrinks by 4 per halving
@@ -118,10 +167,10 @@ LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
         }
 #pragma unroll
         for (int i = 0; i < 18; ++i) Q.v[i] *= scale;
-        Q2 = mul_nn(Q, Q);
+        herm_square(Q, Q2, tr3);
         c1 = retr(Q2) * R(0.5);
     }
-    R c0 = retr_mul(Q, Q2) * (R(1) / R(3));
+    R c0 = tr3 * (R(1) / R(3));
     // f_j = sum_n i^n/n! * coef_j(Q^n); real part from n = 0,2 (mod 4) with
     // signs +,-, imaginary part from n = 1,3 (mod 4) with signs +,-.
     R a = R(1), b = R(0), d = R(0);
@@ -158,13 +207,24 @@ LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
     return o;
 }
 
+// 1/sqrt(n) for n near 1 (a row norm of a nearly unitary matrix): the
+// hardware reciprocal square root refined by one Newton step,
+// y <- y (3 - n y^2) / 2, which squares its relative error (to ~1e-17).
+LGF_DEV double rsqrt_refined(double n) {
+    const double y = __builtin_amdgcn_rsq(n);
+    return y * (1.5 - 0.5 * n * y * y);
+}
+
 // Project onto SU(3) in double precision: Gram-Schmidt on rows 0 and 1,
-// row 2 = conj(row0 x row1).
+// row 2 = conj(row0 x row1).  FAST selects rsqrt_refined for the two
+// normalisations (the per-step projection of the mixed-precision flow);
+// otherwise 1/sqrt, as the CPU oracle (generator, hot start).
+template <bool FAST = false>
 LGF_DEV void reunitarize(M3<double>& m) {
     double n = 0;
 #pragma unroll
     for (int i = 0; i < 6; ++i) n += m.v[i] * m.v[i];
-    n = 1.0 / sqrt(n);
+    n = FAST ? rsqrt_refined(n) : 1.0 / sqrt(n);
 #pragma unroll
     for (int i = 0; i < 6; ++i) m.v[i] *= n;
     double pr = 0, pi = 0;
@@ -183,7 +243,7 @@ LGF_DEV void reunitarize(M3<double>& m) {
     n = 0;
 #pragma unroll
     for (int i = 6; i < 12; ++i) n += m.v[i] * m.v[i];
-    n = 1.0 / sqrt(n);
+    n = FAST ? rsqrt_refined(n) : 1.0 / sqrt(n);
 #pragma unroll
     for (int i = 6; i < 12; ++i) m.v[i] *= n;
 #pragma unroll

@@ -43,7 +43,7 @@ LGF_DEV void stage_link(const S* __restrict__ U, S* __restrict__ Un, S* __restri
     const int s = g.s;
     const M3<C> A = staple_ct<S, C, MU>(U, st, g);
     const M3<C> Umu = load_link<S, C>(U, st, MU, s);
-    const M3<C> Z = ta_scaled(mul_nn(Umu, A), C(-0.5));
+    const M3<C> Z = ta_of_product(Umu, A, C(-0.5));
 
     M3<C> E;
     if (STAGE == 0) {
@@ -76,7 +76,7 @@ LGF_DEV void stage_link(const S* __restrict__ U, S* __restrict__ Un, S* __restri
         M3<double> Ud;
 #pragma unroll
         for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
-        reunitarize(Ud);
+        reunitarize<true>(Ud);
         store_link<S, double>(Un, st, MU, s, Ud);
     } else {
         store_link<S, C>(Un, st, MU, s, Unew);

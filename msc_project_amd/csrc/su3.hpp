@@ -173,4 +173,32 @@ LGF_DEV M3<R> ta_scaled(const M3<R>& W, R s) {
     return o;
 }
 
+// ta_scaled(a * b, s) without the real parts of the product's diagonal,
+// which the projection cancels exactly: 90 FMAs for the product instead of
+// 108.
+template <typename R>
+LGF_DEV M3<R> ta_of_product(const M3<R>& a, const M3<R>& b, R s) {
+    M3<R> W;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const R ar = a.v[2 * (3 * r + k)], ai = a.v[2 * (3 * r + k) + 1];
+                const R br = b.v[2 * (3 * k + c)], bi = b.v[2 * (3 * k + c) + 1];
+                if (r != c) {
+                    x += ar * br;
+                    x -= ai * bi;
+                }
+                y += ar * bi;
+                y += ai * br;
+            }
+            W.v[2 * (3 * r + c)] = x;
+            W.v[2 * (3 * r + c) + 1] = y;
+        }
+    return ta_scaled(W, s);
+}
+
 }  // namespace lgf
