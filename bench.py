@@ -198,7 +198,7 @@ def batched(lgf, ctx):
             "steps": 300, "measure_every": 10, "precision": "fp32"}
 
 
-def serving(lgf, nreq, throughput, seed=4242, policy=1):
+def serving(lgf, nreq, throughput, seed=4242, policy=1, age_rate=0.0):
     """Poisson stream of flow requests (shape uniform in {16^4, 24^4, 32^4,
     32^3x64}, beta in {5.96, 6.05, 6.13, 6.21}, eps 0.01, t uniform in [1, 6],
     measure every 10 steps, seeded random start) at a rate set for ~80%
@@ -216,7 +216,7 @@ def serving(lgf, nreq, throughput, seed=4242, policy=1):
     rate = 0.8 * throughput / mean_work
     gaps = rng.exponential(1.0 / rate, size=nreq)
     srv = lgf.Server(0)
-    srv.set_policy(policy)
+    srv.set_policy(policy, age_rate=age_rate)
     ids = [None] * nreq
 
     def feeder():
@@ -241,6 +241,7 @@ def serving(lgf, nreq, throughput, seed=4242, policy=1):
     ok = all(r["status"] == 0 for r in res)
     lu = sum(4.0 * np.prod(s) * n for s, _, n, _ in reqs)
     return {"requests": nreq, "policy": "shortest_first" if policy == lgf.SERVE_SHORTEST_FIRST else "all",
+            "age_rate": age_rate,
             "arrival_rate_per_s": rate, "all_ok": ok, "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
             "throughput_link_updates_per_s": lu / wall, "wall_s": wall, "busy_s": st["busy_s"],
@@ -377,7 +378,9 @@ def main():
         line["batched_16_uneven"] = batched(lgf, ctx)
         log("batched: %s" % json.dumps(line["batched_16_uneven"]))
     if args.serve_requests > 0:
-        line["serving"] = serving(lgf, args.serve_requests, value, policy=lgf.SERVE_SHORTEST_FIRST)
+        # shortest remaining work first, with one second of waiting credited as
+        # one second of the measured GPU throughput (bounds the wait of large jobs)
+        line["serving"] = serving(lgf, args.serve_requests, value, policy=lgf.SERVE_SHORTEST_FIRST, age_rate=value)
         line["p99_latency_s"] = line["serving"]["p99_latency_s"]
         log("serving: %s" % json.dumps(line["serving"]))
         # the same request stream with every admitted job stepping each round

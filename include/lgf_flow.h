@@ -212,13 +212,15 @@ int lgf_server_submit(lgf_server* s, const lgf_request* req, uint64_t* id);
 int lgf_server_wait(lgf_server* s, uint64_t id, double timeout_s, lgf_result* out);
 /* scheduling policy.  LGF_SERVE_ALL: every admitted job steps in every
  * round (processor sharing).  LGF_SERVE_SHORTEST_FIRST (default): the
- * admitted jobs with the least remaining link-updates step, in that order,
- * until they hold min_sites lattice sites (default 2^20), so short jobs
- * finish first while each launch still fills the GPU; the choice is
+ * admitted jobs step in the order of their remaining link-updates less
+ * age_rate times the seconds since they were submitted (default age_rate 0:
+ * shortest remaining work first), until they hold min_sites lattice sites
+ * (default 2^20), so short jobs finish first while each launch still fills
+ * the GPU, and a positive age_rate lets waiting long jobs in; the choice is
  * revisited whenever a job arrives or finishes.  Results do not depend on
  * the policy. */
 enum lgf_serve_policy { LGF_SERVE_ALL = 0, LGF_SERVE_SHORTEST_FIRST = 1 };
-int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites);
+int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites, double age_rate);
 int lgf_server_now(lgf_server* s, double* now_s);
 /* counters: jobs done, steps executed, link-updates executed, busy seconds */
 int lgf_server_stats(lgf_server* s, uint64_t* jobs_done, uint64_t* batch_steps, double* link_updates,

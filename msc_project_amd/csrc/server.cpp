@@ -12,10 +12,15 @@
 //      choice is revisited when a job arrives or finishes;
 //      run k RK3 steps for those jobs with one set of launches per precision
 //      (a batch over their lattices, whatever their shapes; k = steps to the
-//      nearest measurement or completion, at most kMaxGroup, so new arrivals
-//      are seen within a few steps);
+//      nearest measurement, completion or admission tick);
 //   3. measure the jobs that are due, retire finished jobs and free their
 //      memory, which lets the next queued jobs in.
+// New jobs are admitted only at ticks of kTick global steps (immediately
+// when the server is idle), so jobs with a measurement interval dividing
+// kTick reach their measurements together: the due jobs of a batch are then
+// measured by one sweep over the batch, and a round is never cut short by
+// another job's measurement.  The host does not wait for the GPU between
+// rounds; it waits at measurements, which return results.
 // Every job's arithmetic is independent of the other jobs in the batch, and
 // its reductions run over its own tiles in a fixed order, so a job's results
 // are bitwise the same alone and in any mix.
@@ -37,7 +42,7 @@
 
 namespace {
 
-constexpr int kMaxGroup = 8;      // steps between admission points
+constexpr int kTick = 10;  // admission points are this many global steps apart
 constexpr int kMaxMeasurements = 1024;
 
 struct Job {
@@ -63,6 +68,7 @@ struct lgf_server {
     int max_active = 0;
     int policy = LGF_SERVE_SHORTEST_FIRST;  // guarded by mu
     int64_t min_sites = int64_t(1) << 20;
+    double age_rate = 0.0;
     lgf_ctx* ctx = nullptr;
     std::thread thread;
     std::mutex mu;
@@ -89,6 +95,14 @@ struct lgf_server {
 
 static int64_t volume_of(const int* s) { return (int64_t)s[0] * s[1] * s[2] * s[3]; }
 
+static void record(Job& j, double E, double Q) {
+    if ((int)j.tm.size() < kMaxMeasurements) {
+        j.tm.push_back(j.t);
+        j.E.push_back(E);
+        j.Q.push_back(Q);
+    }
+}
+
 static void measure_job(lgf_server* S, Job& j) {
     double E = 0, Q = 0, Qa = 0;
     int rc = lgf_measure(j.field, j.req.measure_flags, &E, &Q, &Qa, nullptr);
@@ -97,10 +111,58 @@ static void measure_job(lgf_server* S, Job& j) {
         S->set_err(lgf_ctx_last_error(S->ctx));
         return;
     }
-    if ((int)j.tm.size() < kMaxMeasurements) {
-        j.tm.push_back(j.t);
-        j.E.push_back(E);
-        j.Q.push_back(Q);
+    record(j, E, Q);
+}
+
+// Measure the due jobs.  When every stepping job of a precision is due with
+// the same flags, one sweep over that precision's batch measures them all;
+// otherwise each due job's sweep is enqueued first and the results are
+// collected afterwards, so the stream does not wait for the host in between.
+static void measure_due(lgf_server* S, const std::vector<JobP>& stepping, const std::vector<JobP>& due_now,
+                        lgf_batch* const batch[2]) {
+    std::vector<JobP> rest;
+    for (int p = 0; p < 2; ++p) {
+        std::vector<JobP> in_batch, due_p;
+        for (auto& j : stepping)
+            if (j->req.precision == p) in_batch.push_back(j);
+        for (auto& j : due_now)
+            if (j->req.precision == p) due_p.push_back(j);
+        if (due_p.empty()) continue;
+        bool whole = batch[p] && due_p.size() == in_batch.size();
+        for (auto& j : due_p) whole = whole && j->req.measure_flags == due_p[0]->req.measure_flags;
+        if (whole) {
+            std::vector<double> obs(LGF_NUM_OBS * in_batch.size());
+            int rc = lgf_batch_measure_observables(batch[p], due_p[0]->req.measure_flags, obs.data());
+            if (rc == LGF_OK || rc == LGF_E_NAN) {
+                for (size_t i = 0; i < in_batch.size(); ++i)
+                    record(*in_batch[i], obs[LGF_NUM_OBS * i + LGF_OBS_E], obs[LGF_NUM_OBS * i + LGF_OBS_Q]);
+                continue;
+            }
+            S->set_err(lgf_ctx_last_error(S->ctx));
+            for (auto& j : in_batch) j->status = rc;
+            continue;
+        }
+        rest.insert(rest.end(), due_p.begin(), due_p.end());
+    }
+    std::vector<JobP> started;
+    for (auto& j : rest) {
+        int rc = lgf_measure_start(j->field, j->req.measure_flags);
+        if (rc == LGF_OK) {
+            started.push_back(j);
+        } else {
+            j->status = rc;
+            S->set_err(lgf_ctx_last_error(S->ctx));
+        }
+    }
+    for (auto& j : started) {
+        double obs[LGF_NUM_OBS];
+        int rc = lgf_measure_finish(j->field, obs);
+        if (rc == LGF_OK || rc == LGF_E_NAN) {
+            record(*j, obs[LGF_OBS_E], obs[LGF_OBS_Q]);
+        } else {
+            j->status = rc;
+            S->set_err(lgf_ctx_last_error(S->ctx));
+        }
     }
 }
 
@@ -116,15 +178,17 @@ void lgf_server::run() {
     std::vector<JobP> stepping;  // the subset that steps in this round
     lgf_batch* batch[2] = {nullptr, nullptr};
     bool dirty = true;
+    int64_t gstep = 0;  // global steps since the server was last idle
     while (true) {
         std::vector<JobP> admitted;
         {
             std::unique_lock<std::mutex> lk(mu);
             cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
             if (stop) break;
+            if (active.empty()) gstep = 0;
             size_t used = 0;
             for (auto& j : active) used += j->bytes;
-            while (!pending.empty()) {
+            while (gstep % kTick == 0 && !pending.empty()) {
                 JobP j = pending.front();
                 if (used + j->bytes > budget) break;
                 if (max_active > 0 && (int)(active.size() + admitted.size()) >= max_active) break;
@@ -168,15 +232,20 @@ void lgf_server::run() {
         if (dirty) {
             int pol;
             int64_t msites;
+            double arate;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 pol = policy;
                 msites = min_sites;
+                arate = age_rate;
             }
             stepping = active;
             if (pol == LGF_SERVE_SHORTEST_FIRST) {
+                // remaining link-updates, less a credit for the time waited
+                const double tnow = now();
+                auto key = [&](const JobP& j) { return remaining_work(*j) - arate * (tnow - j->submit_s); };
                 std::stable_sort(stepping.begin(), stepping.end(),
-                                 [](const JobP& a, const JobP& b) { return remaining_work(*a) < remaining_work(*b); });
+                                 [&](const JobP& a, const JobP& b) { return key(a) < key(b); });
                 int64_t v = 0;
                 size_t n = 0;
                 while (n < stepping.size() && (n == 0 || v < msites)) v += volume_of(stepping[n++]->req.shape);
@@ -192,8 +261,8 @@ void lgf_server::run() {
             }
             dirty = false;
         }
-        // steps to the nearest event of any stepping job
-        int k = kMaxGroup;
+        // steps to the nearest event of any stepping job, and to the next tick
+        int k = kTick - (int)(gstep % kTick);
         for (auto& j : stepping) {
             int to_end = j->req.nsteps - j->done;
             int every = j->req.measure_every;
@@ -217,13 +286,16 @@ void lgf_server::run() {
                     if (j->req.precision == p) j->status = rc;
             }
         }
-        std::vector<JobP> done_now;
+        gstep += k;
+        std::vector<JobP> done_now, due_now;
         for (auto& j : stepping) {
             for (int s = 0; s < k; ++s) j->t += j->req.eps;  // the reference's t += eps per step
             j->done += k;
-            if (j->status == LGF_OK && due(*j)) measure_job(this, *j);
-            if (j->done >= j->req.nsteps || j->status != LGF_OK) done_now.push_back(j);
+            if (j->status == LGF_OK && due(*j)) due_now.push_back(j);
         }
+        measure_due(this, stepping, due_now, batch);
+        for (auto& j : stepping)
+            if (j->done >= j->req.nsteps || j->status != LGF_OK) done_now.push_back(j);
         std::vector<JobP> still;
         for (auto& j : active)
             if (std::find(done_now.begin(), done_now.end(), j) == done_now.end()) still.push_back(j);
@@ -245,7 +317,6 @@ void lgf_server::run() {
             }
         }
         active.swap(still);
-        lgf_ctx_synchronize(ctx);
         {
             std::lock_guard<std::mutex> lk(mu);
             batch_steps += k;
@@ -367,12 +438,14 @@ extern "C" int lgf_server_wait(lgf_server* s, uint64_t id, double timeout_s, lgf
     return LGF_OK;
 }
 
-extern "C" int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites) {
-    if (!s || (policy != LGF_SERVE_ALL && policy != LGF_SERVE_SHORTEST_FIRST) || min_sites < 0) return LGF_E_ARG;
+extern "C" int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites, double age_rate) {
+    if (!s || (policy != LGF_SERVE_ALL && policy != LGF_SERVE_SHORTEST_FIRST) || min_sites < 0 || !(age_rate >= 0))
+        return LGF_E_ARG;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         s->policy = policy;
         s->min_sites = min_sites;
+        s->age_rate = age_rate;
     }
     s->cv_work.notify_all();
     return LGF_OK;

@@ -98,7 +98,7 @@ SIGNATURES = {
     "lgf_server_submit": ([_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "lgf_server_wait": ([_vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(Result)], ctypes.c_int),
     "lgf_server_now": ([_vp, _dp], ctypes.c_int),
-    "lgf_server_set_policy": ([_vp, ctypes.c_int, ctypes.c_int64], ctypes.c_int),
+    "lgf_server_set_policy": ([_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_double], ctypes.c_int),
     "lgf_server_stats": ([_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _dp, _dp],
                          ctypes.c_int),
     "lgf_server_last_error": ([_vp], ctypes.c_char_p),
@@ -394,8 +394,9 @@ class Server:
         self.h = h
         _live.add(self)
 
-    def set_policy(self, policy, min_sites=1 << 20):
-        self._check(lib().lgf_server_set_policy(self.h, int(policy), int(min_sites)), "set_policy")
+    def set_policy(self, policy, min_sites=1 << 20, age_rate=0.0):
+        """age_rate: link-updates of remaining work credited per second waited"""
+        self._check(lib().lgf_server_set_policy(self.h, int(policy), int(min_sites), float(age_rate)), "set_policy")
 
     def _check(self, rc, what):
         if rc:
