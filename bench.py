@@ -139,11 +139,8 @@ def run_job(lgf, ctx, src, prec, steps, warmup, ev):
 
 
 def t0_of(t, E):
-    y = t ** 2 * E
-    for i in range(1, len(t)):
-        if y[i - 1] < 0.3 <= y[i]:
-            return t[i - 1] + (0.3 - y[i - 1]) * (t[i] - t[i - 1]) / (y[i] - y[i - 1])
-    return None
+    from msc_project_amd import lgf
+    return lgf.flow_scale_t0(t, E, 0.3)
 
 
 def accuracy(r64, r32):

@@ -157,6 +157,13 @@ int lgf_measure(lgf_field* f, int flags, double* E, double* Q, double* Qabs, dou
  * MeasureFlowedGaugeConfigurations (LGF/LGF.cpp:228-255) */
 int lgf_measure_observables(lgf_field* f, int flags, double* obs, double* qraw);
 int lgf_diagnostics(lgf_field* f, double* avg_plaquette, double* unitarity_max, double* unitarity_mean);
+/* flow-time scale t0 of a measured series: the t at which t^2 E(t) first
+ * reaches `target` (0.3 defines t0, doc/thesis.tex:1214-1216; the reference
+ * leaves it to its analysis scripts, DataAnalysis_and_Plotting/
+ * t0_extrapolation.py), by linear interpolation of t^2 E between the two
+ * bracketing measurements.  t ascending, n >= 2.  LGF_E_STATE if the series
+ * never reaches the target.  Host-side; needs no context. */
+int lgf_flow_scale_t0(const double* t, const double* E, int n, double target, double* t0);
 
 /* ---- many lattices (any shapes) in one set of launches ----------------- */
 int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfields, lgf_batch** out);

@@ -399,6 +399,19 @@ extern "C" int lgf_field_heatbath(lgf_field* f, double beta, uint64_t seed, int 
     return LGF_OK;
 }
 
+extern "C" int lgf_flow_scale_t0(const double* t, const double* E, int n, double target, double* t0) {
+    if (!t || !E || !t0 || n < 2) return LGF_E_ARG;
+    for (int i = 0; i + 1 < n; ++i) {
+        if (!(t[i + 1] > t[i])) return LGF_E_ARG;
+        const double a = t[i] * t[i] * E[i] - target, b = t[i + 1] * t[i + 1] * E[i + 1] - target;
+        if (a < 0.0 && b >= 0.0) {
+            *t0 = t[i] + (t[i + 1] - t[i]) * (-a) / (b - a);
+            return LGF_OK;
+        }
+    }
+    return LGF_E_STATE;
+}
+
 extern "C" int lgf_diagnostics(lgf_field* f, double* plaq, double* umax, double* umean) {
     if (!f) return LGF_E_ARG;
     lgf_ctx* ctx = f->ctx;

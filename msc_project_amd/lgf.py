@@ -98,6 +98,7 @@ SIGNATURES = {
     "lgf_server_submit": ([_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "lgf_server_wait": ([_vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(Result)], ctypes.c_int),
     "lgf_server_now": ([_vp, _dp], ctypes.c_int),
+    "lgf_flow_scale_t0": ([_dp, _dp, ctypes.c_int, ctypes.c_double, _dp], ctypes.c_int),
     "lgf_server_set_policy": ([_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_double], ctypes.c_int),
     "lgf_server_stats": ([_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _dp, _dp],
                          ctypes.c_int),
@@ -327,6 +328,20 @@ class Field:
             self.close()
         except Exception:
             pass
+
+
+def flow_scale_t0(t, E, target=0.3):
+    """t at which t^2 E(t) first reaches `target`, by linear interpolation of
+    t^2 E (lgf_flow_scale_t0); None if the series never reaches it."""
+    t = np.ascontiguousarray(t, dtype=np.float64)
+    E = np.ascontiguousarray(E, dtype=np.float64)
+    out = ctypes.c_double()
+    rc = lib().lgf_flow_scale_t0(_p(t), _p(E), len(t), float(target), ctypes.byref(out))
+    if rc == 7:
+        return None
+    if rc:
+        raise LgfError(rc, "flow_scale_t0")
+    return out.value
 
 
 class Batch:

@@ -4,6 +4,7 @@ import ctypes
 import os
 import re
 
+import numpy as np
 import pytest
 
 from msc_project_amd import lgf
@@ -50,3 +51,18 @@ def test_context_without_gpu_fails_loudly():
     code = "from msc_project_amd import lgf\ntry:\n    lgf.Context(0)\n    print('GPU')\nexcept lgf.LgfError as e:\n    print('ERR', e.code)\n"
     out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, cwd=ROOT, timeout=120)
     assert out.stdout.strip() in ("GPU", "ERR 5")
+
+
+def test_flow_scale_t0_interpolates_t2E():
+    """t0 from t^2 E = 0.3: linear interpolation between the bracketing points
+    (checked against a closed form) and None when the series stays below"""
+    t = np.linspace(0.0, 5.0, 101)
+    E = 0.04 + 0.0 * t  # t^2 E = 0.04 t^2 crosses 0.3 at t = sqrt(7.5)
+    t0 = lgf.flow_scale_t0(t, E)
+    i = np.searchsorted(t ** 2 * E, 0.3)
+    a, b = t[i - 1] ** 2 * E[i - 1] - 0.3, t[i] ** 2 * E[i] - 0.3
+    assert abs(t0 - (t[i - 1] + (t[i] - t[i - 1]) * (-a) / (b - a))) < 1e-15
+    assert abs(t0 - np.sqrt(7.5)) < 2e-3
+    assert lgf.flow_scale_t0(t[:20], E[:20]) is None
+    with pytest.raises(lgf.LgfError):
+        lgf.flow_scale_t0(t[::-1], E)
