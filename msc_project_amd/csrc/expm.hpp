@@ -10,7 +10,7 @@
 // exp_ah_series: the same f0 + f1 Q + f2 Q^2 decomposition, but with the f_j
 //   summed from the exponential series using the Cayley-Hamilton recurrence
 //   Q^{n+1} = c0 d_n + (a_n + c1 d_n) Q + b_n Q^2 for Q^n = a_n + b_n Q + d_n Q^2.
-//   No transcendental functions; truncation below 1e-10 (see the bound at
+//   No transcendental functions; truncation below 1e-9 (see the bound at
 //   exp_ah_series).  Larger exponents are scaled and squared.  Used by the
 //   mixed-precision path.
 #pragma once
@@ -140,13 +140,14 @@ LGF_DEV void herm_square(const M3<R>& Q, M3<R>& Q2, R& tr3) {
 // Power series through the Cayley-Hamilton recurrence, real type R.
 // Truncated after Q^(NTERMS-1): with c1 <= C1MAX every eigenvalue satisfies
 // |lambda| <= sqrt(4 c1 / 3) and the first omitted term is below
-// |lambda|^NTERMS / NTERMS!.  The defaults (10 terms, c1 <= 0.15, so
-// |lambda| <= 0.45) bound it by 1e-10, far below FP32 rounding.  Larger
-// exponents (rare: a flow step moves a link by eps*|Z|) are scaled by 2^-k
-// into that range and the result squared k times, all in precision R.
-template <typename R, int NTERMS = 10>
+// |lambda|^NTERMS / NTERMS!.  The defaults (7 terms, c1 <= 0.02, so
+// |lambda| <= 0.164) bound it by 7e-10, far below FP32 rounding; a flow
+// step's exponent eps*Z has c1 ~ 1e-4..1e-3 at eps = 0.01.  Larger
+// exponents are scaled by 2^-k into that range and the result squared k
+// times, all in precision R.
+template <typename R, int NTERMS = 7>
 LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
-    constexpr float C1MAX = 0.15f;
+    constexpr float C1MAX = 0.02f;
     M3<R> Q;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {

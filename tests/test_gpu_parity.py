@@ -80,8 +80,10 @@ def test_measure_matches_oracle(ctx, shape):
     assert np.max(np.abs(qraw - qo_raw)) < 1e-12 * np.max(np.abs(qo_raw))
 
 
-def test_flow_matches_oracle_links(ctx):
-    shape = (4, 6, 4, 4)
+@pytest.mark.parametrize("shape", [(4, 6, 4, 4), (2, 4, 6, 4), (6, 4, 4, 2)])
+def test_flow_matches_oracle_links(ctx, shape):
+    """FP64 links after 7 steps equal the oracle's; extents of 2 make x+mu and
+    x-mu the same site, which the staple's neighbour offsets must respect"""
     U = O.generate(shape, 6.0, seed=77, start="hot", sweeps=2)
     f = lgf.Field(ctx, shape, lgf.FP64)
     f.upload(U)
@@ -274,6 +276,24 @@ def test_bad_shape_rejected(ctx):
     with pytest.raises(lgf.LgfError) as e:
         lgf.Field(ctx, (4, 5, 4, 4), lgf.FP64)
     assert e.value.code == 2
+
+
+def test_fp32_flow_tracks_oracle_on_hot_start(ctx):
+    """the mixed-precision flow (7-term series exponential, FP64 projection
+    per step) stays within FP32 rounding of the FP64 oracle on a rough hot
+    start, where the exponents are largest, also at eps = 0.05 where the
+    scaling-and-squaring branch is taken"""
+    shape = (4, 4, 6, 4)
+    U = O.generate(shape, 6.0, seed=91, start="hot", sweeps=0)
+    for eps in (0.01, 0.05):
+        f = lgf.Field(ctx, shape, lgf.FP32)
+        f.upload(U)
+        f.flow(eps, 5)
+        V = U.copy()
+        for _ in range(5):
+            O.rk3_step(shape, V, eps)
+        assert np.max(np.abs(f.download() - V)) < 2e-5, eps
+        f.close()
 
 
 def test_fp32_tracks_fp64(ctx, config1):
