@@ -70,7 +70,7 @@ LGF_DEV void stage_link(const S* __restrict__ U, S* __restrict__ Un, S* __restri
     } else {
         ex = exp_ah_series<C>(E);
     }
-    M3<C> Unew = mul_nn(ex, Umu);
+    M3<C> Unew = su3_nn(ex, Umu);
     if constexpr (EXPMODE == 1 && STAGE == 2) {
         // once per RK3 step: project back onto SU(3) in double precision
         M3<double> Ud;
@@ -209,8 +209,8 @@ LGF_DEV void diag_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const Plane
     const int sc = it ? s + nb + mb : s;
     const int scn = it ? s + mb : s + nf;  // c + n nu
     const int scm = it ? s + nb : s + mf;  // c + m mu
-    M3<C> P = mul_nn(line<N>(L, nu, sc), line<M>(L, mu, scn));
-    const M3<C> R = mul_nn(line<M>(L, mu, sc), line<N>(L, nu, scm));
+    M3<C> P = su3_nn(line<N>(L, nu, sc), line<M>(L, mu, scn));
+    const M3<C> R = su3_nn(line<M>(L, mu, sc), line<N>(L, nu, scm));
     if (w != C(1)) scale(P, w);
     if (it == 0) mul_nd_acc(acc, P, R);
     else mul_dn_acc(acc, R, P);
@@ -227,8 +227,8 @@ LGF_DEV void side_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const Plane
     const int m2 = o.m[3 - L2], n2 = o.n[3 - L2], m1 = o.m[3 + L1], n1 = o.n[3 + L1];
     const int zb = it ? m2 : n2;  // z = x - l2 beta
     const int za = it ? n1 : m1;  // + l1 alpha
-    const M3<C> H = mul_nn(line<L1>(L, a, s + zb), line<L2>(L, b, s + zb + za));
-    M3<C> T = mul_nd(line<L1>(L, a, s), H);
+    const M3<C> H = su3_nn(line<L1>(L, a, s + zb), line<L2>(L, b, s + zb + za));
+    M3<C> T = su3_nd(line<L1>(L, a, s), H);
     scale(T, it ? -w : w);
     mul_nn_acc(acc, T, line<L2>(L, b, s + zb));
 }
@@ -336,11 +336,11 @@ __global__ __launch_bounds__(256) void k_lines(const FieldDesc* __restrict__ fds
     if (s >= d.vol) return;
     const S* __restrict__ U = (const S*)d.links[ping ^ d.flip];
     const Geo g = make_geo(d, s);
-    const M3<C> L2 = mul_nn(load_link<S, C>(U, d.stride, mu, s), load_link<S, C>(U, d.stride, mu, s + nb_off(g, mu, +1)));
+    const M3<C> L2 = su3_nn(load_link<S, C>(U, d.stride, mu, s), load_link<S, C>(U, d.stride, mu, s + nb_off(g, mu, +1)));
     const int li = (d.tile0 - t0) * kTileSites + s;
     store_link<C, C>(lines, G, mu, li, L2);
     store_link<C, C>(lines + 4 * 18 * G, G, mu, li,
-                     mul_nn(L2, load_link<S, C>(U, d.stride, mu, s + wrap_offset(g, mu, 2))));
+                     su3_nn(L2, load_link<S, C>(U, d.stride, mu, s + wrap_offset(g, mu, 2))));
 }
 
 // One thread per (site, plane).  The six planes are taken in the order of

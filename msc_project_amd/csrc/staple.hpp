@@ -37,36 +37,6 @@ LGF_DEV M3<C> staple(const S* __restrict__ U, int64_t st, const Geo& g, int mu) 
     return A;
 }
 
-// Product of two SU(3) matrices: rows 0 and 1 by the full product, row 2 as
-// the conjugated cross product of rows 0 and 1 (exact for SU(3)).
-template <typename R>
-LGF_DEV M3<R> mul_nn_su3(const M3<R>& a, const M3<R>& b) {
-    M3<R> o;
-#pragma unroll
-    for (int r = 0; r < 2; ++r)
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            R x = R(0), y = R(0);
-#pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                LGF_CMAC(x, y, a.v[2 * (3 * r + k)], a.v[2 * (3 * r + k) + 1], b.v[2 * (3 * k + c)],
-                         b.v[2 * (3 * k + c) + 1])
-            }
-            o.v[2 * (3 * r + c)] = x;
-            o.v[2 * (3 * r + c) + 1] = y;
-        }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
-        R x = R(0), y = R(0);
-        LGF_CMAC(x, y, o.v[2 * c1], o.v[2 * c1 + 1], o.v[6 + 2 * c2], o.v[7 + 2 * c2])
-        LGF_CMAC(x, y, -o.v[2 * c2], -o.v[2 * c2 + 1], o.v[6 + 2 * c1], o.v[7 + 2 * c1])
-        o.v[12 + 2 * c] = x;
-        o.v[13 + 2 * c] = -y;
-    }
-    return o;
-}
-
 // Half-staple product U V of two links.  The mixed-precision path (FP32
 // arithmetic, links projected onto SU(3) every step) forms it from two rows
 // and their cross product, 96 instead of 108 FMAs; the FP64 path keeps the
@@ -74,10 +44,7 @@ LGF_DEV M3<R> mul_nn_su3(const M3<R>& a, const M3<R>& b) {
 // drift from SU(3) by up to ~1e-14 over a flow.
 template <typename R>
 LGF_DEV M3<R> mul_halfstaple(const M3<R>& a, const M3<R>& b) {
-    if constexpr (sizeof(R) == 4)
-        return mul_nn_su3(a, b);
-    else
-        return mul_nn(a, b);
+    return su3_nn(a, b);
 }
 
 // The same sum with the direction a compile-time constant: every neighbour

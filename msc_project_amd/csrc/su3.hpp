@@ -173,6 +173,83 @@ LGF_DEV M3<R> ta_scaled(const M3<R>& W, R s) {
     return o;
 }
 
+// Products of two SU(3) matrices whose third row is rebuilt as the
+// conjugated cross product of the first two (exact for SU(3)): 96 FMAs
+// instead of 108.
+template <typename R>
+LGF_DEV void su3_third_row(M3<R>& o) {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        R x = R(0), y = R(0);
+        LGF_CMAC(x, y, o.v[2 * c1], o.v[2 * c1 + 1], o.v[6 + 2 * c2], o.v[7 + 2 * c2])
+        LGF_CMAC(x, y, -o.v[2 * c2], -o.v[2 * c2 + 1], o.v[6 + 2 * c1], o.v[7 + 2 * c1])
+        o.v[12 + 2 * c] = x;
+        o.v[13 + 2 * c] = -y;
+    }
+}
+
+// a b from rows 0, 1 and their cross product
+template <typename R>
+LGF_DEV M3<R> mul_nn_su3(const M3<R>& a, const M3<R>& b) {
+    M3<R> o;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                LGF_CMAC(x, y, a.v[2 * (3 * r + k)], a.v[2 * (3 * r + k) + 1], b.v[2 * (3 * k + c)],
+                         b.v[2 * (3 * k + c) + 1])
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+    su3_third_row(o);
+    return o;
+}
+
+// a b^dag from rows 0, 1 and their cross product
+template <typename R>
+LGF_DEV M3<R> mul_nd_su3(const M3<R>& a, const M3<R>& b) {
+    M3<R> o;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                LGF_CMAC(x, y, a.v[2 * (3 * r + k)], a.v[2 * (3 * r + k) + 1], b.v[2 * (3 * c + k)],
+                         -b.v[2 * (3 * c + k) + 1])
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+    su3_third_row(o);
+    return o;
+}
+
+// Products of SU(3) matrices in the arithmetic of the mixed-precision path
+// (float: links projected onto SU(3) every flow step) use the cross-product
+// row; in double, the parity path, the full product as the reference.
+template <typename R>
+LGF_DEV M3<R> su3_nn(const M3<R>& a, const M3<R>& b) {
+    if constexpr (sizeof(R) == 4)
+        return mul_nn_su3(a, b);
+    else
+        return mul_nn(a, b);
+}
+
+template <typename R>
+LGF_DEV M3<R> su3_nd(const M3<R>& a, const M3<R>& b) {
+    if constexpr (sizeof(R) == 4)
+        return mul_nd_su3(a, b);
+    else
+        return mul_nd(a, b);
+}
+
 // ta_scaled(a * b, s) without the real parts of the product's diagonal,
 // which the projection cancels exactly: 90 FMAs for the product instead of
 // 108.
