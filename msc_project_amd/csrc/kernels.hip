@@ -212,8 +212,9 @@ LGF_DEV void diag_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const Plane
     M3<C> P = su3_nn(line<N>(L, nu, sc), line<M>(L, mu, scn));
     const M3<C> R = su3_nn(line<M>(L, mu, sc), line<N>(L, nu, scm));
     if (w != C(1)) scale(P, w);
-    if (it == 0) mul_nd_acc(acc, P, R);
-    else mul_dn_acc(acc, R, P);
+    // the clover sums enter only through their traceless anti-Hermitian part
+    if (it == 0) mul_nd_acc_ta(acc, P, R);
+    else mul_dn_acc_ta(acc, R, P);
 }
 
 // side quadrants: it = 0 is (+mu,-nu) with (alpha, beta) = (mu, nu); it = 1
@@ -230,7 +231,7 @@ LGF_DEV void side_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const Plane
     const M3<C> H = su3_nn(line<L1>(L, a, s + zb), line<L2>(L, b, s + zb + za));
     M3<C> T = su3_nd(line<L1>(L, a, s), H);
     scale(T, it ? -w : w);
-    mul_nn_acc(acc, T, line<L2>(L, b, s + zb));
+    mul_nn_acc_ta(acc, T, line<L2>(L, b, s + zb));
 }
 
 // F = 1/2 (-i) [ (C - C^dag) - 1/3 Tr(C - C^dag) ] (LGF/Field/SU3_field.cpp:575-579)

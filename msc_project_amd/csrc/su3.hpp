@@ -173,6 +173,40 @@ LGF_DEV M3<R> ta_scaled(const M3<R>& W, R s) {
     return o;
 }
 
+// o += a b, o += a b^dag, o += a^dag b without the real parts of the
+// diagonal, for sums whose only use is their traceless anti-Hermitian part
+// (ta_scaled), where those parts cancel exactly whatever their values.
+#define LGF_TA_ACC_BODY(AIDX, BIDX, CONJA, CONJB)                                              \
+    _Pragma("unroll") for (int r = 0; r < 3; ++r) _Pragma("unroll") for (int c = 0; c < 3; ++c) { \
+        R x = o.v[2 * (3 * r + c)], y = o.v[2 * (3 * r + c) + 1];                               \
+        _Pragma("unroll") for (int k = 0; k < 3; ++k) {                                          \
+            const R ar = a.v[AIDX], ai = CONJA a.v[AIDX + 1];                                     \
+            const R br = b.v[BIDX], bi = CONJB b.v[BIDX + 1];                                     \
+            if (r != c) {                                                                         \
+                x += ar * br;                                                                     \
+                x -= ai * bi;                                                                     \
+            }                                                                                     \
+            y += ar * bi;                                                                         \
+            y += ai * br;                                                                         \
+        }                                                                                         \
+        o.v[2 * (3 * r + c)] = x;                                                                 \
+        o.v[2 * (3 * r + c) + 1] = y;                                                             \
+    }
+
+template <typename R>
+LGF_DEV void mul_nn_acc_ta(M3<R>& o, const M3<R>& a, const M3<R>& b) {
+    LGF_TA_ACC_BODY(2 * (3 * r + k), 2 * (3 * k + c), , )
+}
+template <typename R>
+LGF_DEV void mul_nd_acc_ta(M3<R>& o, const M3<R>& a, const M3<R>& b) {
+    LGF_TA_ACC_BODY(2 * (3 * r + k), 2 * (3 * c + k), , -)
+}
+template <typename R>
+LGF_DEV void mul_dn_acc_ta(M3<R>& o, const M3<R>& a, const M3<R>& b) {
+    LGF_TA_ACC_BODY(2 * (3 * k + r), 2 * (3 * k + c), -, )
+}
+#undef LGF_TA_ACC_BODY
+
 // Products of two SU(3) matrices whose third row is rebuilt as the
 // conjugated cross product of the first two (exact for SU(3)): 96 FMAs
 // instead of 108.
