@@ -210,7 +210,10 @@ typedef struct {
     double Q[1024];
 } lgf_result;
 
-/* budget_bytes: device memory the server may hold for jobs (0 = 80% of free);
+/* budget_bytes: device memory the server may hold (0 = 80% of free): the
+ * jobs' fields (lgf_field_footprint) plus the measurement scratch, kept
+ * aside as max(1 GiB, the line products of the largest admitted lattice:
+ * 576 B per site for FP32 clover arithmetic, 1152 B for FP64);
  * max_active: jobs flowing at once (0 = unlimited within the budget) */
 int lgf_server_create(int device, size_t budget_bytes, int max_active, lgf_server** out);
 int lgf_server_destroy(lgf_server* s);

@@ -4,7 +4,9 @@
 // another from a CLI (LGF/LGF.cpp:163-196).  Here flow jobs arrive at any
 // time; one scheduler thread owns the GPU context and loops:
 //   1. admit queued jobs, first come first served, while their device
-//      footprint (2 link buffers + RK3 accumulator) fits the budget;
+//      footprint (2 link buffers + RK3 accumulator) fits the budget, less
+//      the measurement scratch (line products: up to 1 GiB per group of
+//      lattices, or one lattice's worth if that is more);
 //   2. choose the jobs that step: every admitted job (LGF_SERVE_ALL), or
 //      (LGF_SERVE_SHORTEST_FIRST) the admitted jobs with the least remaining
 //      link-updates, in that order, until they hold min_sites lattice sites,
@@ -50,6 +52,7 @@ struct Job {
     lgf_request req{};
     std::vector<double> links;  // copy of the caller's initial links (may be empty)
     size_t bytes = 0;
+    size_t line_bytes = 0;  // line products of one measurement of this job
     lgf_field* field = nullptr;
     int done = 0;
     double t = 0.0;
@@ -94,6 +97,11 @@ struct lgf_server {
 };
 
 static int64_t volume_of(const int* s) { return (int64_t)s[0] * s[1] * s[2] * s[3]; }
+
+// device scratch a measurement of these jobs can hold: the line products of
+// a group of lattices (the context's default cap) or of the largest one
+constexpr size_t kMeasureScratchCap = size_t(1) << 30;
+static size_t measure_reserve(size_t largest_line_bytes) { return std::max(kMeasureScratchCap, largest_line_bytes); }
 
 static void record(Job& j, double E, double Q) {
     if ((int)j.tm.size() < kMaxMeasurements) {
@@ -186,14 +194,18 @@ void lgf_server::run() {
             cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
             if (stop) break;
             if (active.empty()) gstep = 0;
-            size_t used = 0;
-            for (auto& j : active) used += j->bytes;
+            size_t used = 0, lines = 0;
+            for (auto& j : active) {
+                used += j->bytes;
+                lines = std::max(lines, j->line_bytes);
+            }
             while (gstep % kTick == 0 && !pending.empty()) {
                 JobP j = pending.front();
-                if (used + j->bytes > budget) break;
+                if (used + j->bytes + measure_reserve(std::max(lines, j->line_bytes)) > budget) break;
                 if (max_active > 0 && (int)(active.size() + admitted.size()) >= max_active) break;
                 pending.pop_front();
                 used += j->bytes;
+                lines = std::max(lines, j->line_bytes);
                 admitted.push_back(j);
             }
         }
@@ -388,15 +400,19 @@ extern "C" int lgf_server_submit(lgf_server* s, const lgf_request* req, uint64_t
     int rc = lgf_field_footprint(req->shape, req->precision, &bytes);
     if (rc) return rc;
     if (req->nsteps < 0 || req->measure_every < 0 || !(req->eps == req->eps)) return LGF_E_ARG;
-    if (bytes > s->budget) {
+    const bool fp64_lines = req->precision == LGF_FP64 || (req->measure_flags & LGF_MEASURE_FP64);
+    const size_t line_bytes = size_t(volume_of(req->shape)) * 2 * 4 * 18 * (fp64_lines ? 8 : 4);
+    if (bytes + measure_reserve(line_bytes) > s->budget) {
         std::lock_guard<std::mutex> lk(s->mu);
-        s->err = "request needs " + std::to_string(bytes) + " bytes of device memory, budget is " +
+        s->err = "request needs " + std::to_string(bytes) + " bytes of device memory and " +
+                 std::to_string(measure_reserve(line_bytes)) + " of measurement scratch, budget is " +
                  std::to_string(s->budget);
         return LGF_E_ADMISSION;
     }
     auto j = std::make_shared<Job>();
     j->req = *req;
     j->bytes = bytes;
+    j->line_bytes = line_bytes;
     if (req->links) j->links.assign(req->links, req->links + (size_t)volume_of(req->shape) * 72);
     j->req.links = nullptr;
     {

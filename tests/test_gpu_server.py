@@ -61,7 +61,8 @@ def test_server_results_same_alone_and_in_mix(policy, min_sites):
 def test_server_admission_by_footprint():
     shape = (8, 8, 8, 8)
     one = lgf.footprint(shape, lgf.FP32)
-    srv = lgf.Server(0, budget_bytes=int(2.5 * one))  # two jobs fit at a time
+    reserve = 1 << 30  # the server keeps the measurement scratch (line products) aside
+    srv = lgf.Server(0, budget_bytes=reserve + int(2.5 * one))  # two jobs fit at a time
     with pytest.raises(lgf.LgfError) as e:
         srv.submit((16, 16, 16, 16), 0.01, 5, 5)
     assert e.value.code == 8
