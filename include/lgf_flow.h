@@ -93,7 +93,11 @@ const char* lgf_ctx_last_error(const lgf_ctx* ctx);
 int lgf_ctx_stream(lgf_ctx* ctx, void** hip_stream);
 int lgf_ctx_synchronize(lgf_ctx* ctx);
 /* RK3 stage-kernel variant (register budget x workgroup shape) used by this
- * context for one precision; 0 = default */
+ * context for one precision; 0 = default.  Mixed-precision variants below
+ * LGF_FP32_SHARED_PLAQUETTE_VARIANTS compute each plaquette once for the
+ * four links of a site; the others sum each link's staples in the FP64
+ * path's order.  The two groups agree to FP32 rounding, not bit for bit. */
+#define LGF_FP32_SHARED_PLAQUETTE_VARIANTS 3
 int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
 int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
 int lgf_num_stage_variants(int precision, int* n);

@@ -47,6 +47,18 @@ LGF_DEV M3<R> mul_halfstaple(const M3<R>& a, const M3<R>& b) {
     return su3_nn(a, b);
 }
 
+// lower term of the (MU,NU) plane: A += (U_mu(x-nu) U_nu(x-nu+mu))^dag U_nu(x-nu)
+template <typename S, typename C, int MU, int NU>
+LGF_DEV void lower_term(M3<C>& A, const S* __restrict__ U, int64_t st, const Geo& g, int fmu) {
+    const int s = g.s;
+    const int bnu = nb_off(g, NU, -1);
+    M3<C> a = load_link<S, C>(U, st, MU, s + bnu);
+    M3<C> b = load_link<S, C>(U, st, NU, s + bnu + fmu);
+    M3<C> tt = mul_halfstaple(a, b);
+    M3<C> c = load_link<S, C>(U, st, NU, s + bnu);
+    mul_dn_acc(A, tt, c);
+}
+
 // The same sum with the direction a compile-time constant: every neighbour
 // offset and plane base is then a constant expression of the site's
 // coordinates, with no selects on a run-time direction.  The terms are added
@@ -54,7 +66,7 @@ LGF_DEV M3<R> mul_halfstaple(const M3<R>& a, const M3<R>& b) {
 template <typename S, typename C, int MU, int NU>
 LGF_DEV void staple_term(M3<C>& A, const S* __restrict__ U, int64_t st, const Geo& g, int fmu) {
     const int s = g.s;
-    const int fnu = nb_off(g, NU, +1), bnu = nb_off(g, NU, -1);
+    const int fnu = nb_off(g, NU, +1);
     {
         M3<C> a = load_link<S, C>(U, st, NU, s);
         M3<C> b = load_link<S, C>(U, st, MU, s + fnu);
@@ -62,13 +74,7 @@ LGF_DEV void staple_term(M3<C>& A, const S* __restrict__ U, int64_t st, const Ge
         M3<C> c = load_link<S, C>(U, st, NU, s + fmu);
         mul_nd_acc(A, c, tt);
     }
-    {
-        M3<C> a = load_link<S, C>(U, st, MU, s + bnu);
-        M3<C> b = load_link<S, C>(U, st, NU, s + bnu + fmu);
-        M3<C> tt = mul_halfstaple(a, b);
-        M3<C> c = load_link<S, C>(U, st, NU, s + bnu);
-        mul_dn_acc(A, tt, c);
-    }
+    lower_term<S, C, MU, NU>(A, U, st, g, fmu);
 }
 
 template <typename S, typename C, int MU>

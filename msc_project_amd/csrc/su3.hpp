@@ -312,4 +312,38 @@ LGF_DEV M3<R> ta_of_product(const M3<R>& a, const M3<R>& b, R s) {
     return ta_scaled(W, s);
 }
 
+// ta_scaled(a * b^dag, s) without the real parts of the product's diagonal
+template <typename R>
+LGF_DEV M3<R> ta_of_product_nd(const M3<R>& a, const M3<R>& b, R s) {
+    M3<R> W;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const R ar = a.v[2 * (3 * r + k)], ai = a.v[2 * (3 * r + k) + 1];
+                const R br = b.v[2 * (3 * c + k)], bi = b.v[2 * (3 * c + k) + 1];
+                if (r != c) {
+                    x += ar * br;
+                    x += ai * bi;
+                }
+                y += ai * br;
+                y -= ar * bi;
+            }
+            W.v[2 * (3 * r + c)] = x;
+            W.v[2 * (3 * r + c) + 1] = y;
+        }
+    return ta_scaled(W, s);
+}
+
+// The nine reals that hold a traceless anti-Hermitian matrix exactly:
+// (Im m00, Im m11, Im m22, Re m01, Im m01, Re m02, Im m02, Re m12, Im m12)
+template <typename R>
+LGF_DEV void ta_pack(const M3<R>& m, R* a) {
+    a[0] = m.v[1], a[1] = m.v[9], a[2] = m.v[17];
+    a[3] = m.v[2], a[4] = m.v[3], a[5] = m.v[4], a[6] = m.v[5], a[7] = m.v[10], a[8] = m.v[11];
+}
+
 }  // namespace lgf

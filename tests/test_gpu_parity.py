@@ -278,13 +278,15 @@ def test_bad_shape_rejected(ctx):
     assert e.value.code == 2
 
 
-def test_fp32_flow_tracks_oracle_on_hot_start(ctx):
+@pytest.mark.parametrize("variant", [0, lgf.FP32_SHARED_PLAQUETTE_VARIANTS])
+def test_fp32_flow_tracks_oracle_on_hot_start(ctx, variant):
     """the mixed-precision flow (7-term series exponential, FP64 projection
     per step) stays within FP32 rounding of the FP64 oracle on a rough hot
     start, where the exponents are largest, also at eps = 0.05 where the
     scaling-and-squaring branch is taken"""
     shape = (4, 4, 6, 4)
     U = O.generate(shape, 6.0, seed=91, start="hot", sweeps=0)
+    ctx.set_stage_variant(lgf.FP32, variant)
     for eps in (0.01, 0.05):
         f = lgf.Field(ctx, shape, lgf.FP32)
         f.upload(U)
@@ -294,11 +296,14 @@ def test_fp32_flow_tracks_oracle_on_hot_start(ctx):
             O.rk3_step(shape, V, eps)
         assert np.max(np.abs(f.download() - V)) < 2e-5, eps
         f.close()
+    ctx.set_stage_variant(lgf.FP32, 0)
 
 
-def test_fp32_tracks_fp64(ctx, config1):
+@pytest.mark.parametrize("variant", [0, lgf.FP32_SHARED_PLAQUETTE_VARIANTS])
+def test_fp32_tracks_fp64(ctx, config1, variant):
     """mixed precision vs the FP64 path on the config-1 inputs"""
     d, shape, cfgs = config1
+    ctx.set_stage_variant(lgf.FP32, variant)
     U = cfgs[0]
     f64 = lgf.Field(ctx, shape, lgf.FP64)
     f32 = lgf.Field(ctx, shape, lgf.FP32)
@@ -308,13 +313,17 @@ def test_fp32_tracks_fp64(ctx, config1):
     _, E32, Q32, _ = flow_measure(f32, d["eps"], d["steps"], d["every"], flags=lgf.MEASURE_FP64)
     dE = np.abs(t ** 2 * (E32 - E64)) / np.abs(t ** 2 * E64 + 1e-300)
     print("fp32: max rel d(t^2E) %.2e  max |dQ| %.2e" % (dE[1:].max(), np.abs(Q32 - Q64).max()))
+    ctx.set_stage_variant(lgf.FP32, 0)
     assert np.abs(Q32 - Q64).max() < 1e-4
     assert dE[1:].max() < 1e-5
 
 
-def test_stage_variants_agree_and_autotune(ctx):
-    """every stage-kernel variant computes the same links; the autotuner picks one of them"""
-    shape = (8, 4, 6, 4)
+@pytest.mark.parametrize("shape", [(8, 4, 6, 4), (6, 4, 2, 2)])
+def test_stage_variants_agree_and_autotune(ctx, shape):
+    """every stage-kernel variant computes the same links (the mixed-precision
+    shared-plaquette and per-link kernels, which sum the staples in different
+    orders, to FP32 rounding); the autotuner picks one of them.  6x4x2x2 has
+    a ragged last tile (96 sites)."""
     U = O.generate(shape, 6.0, seed=31, start="hot", sweeps=1)
     for prec in (lgf.FP64, lgf.FP32):
         outs = []
@@ -325,8 +334,13 @@ def test_stage_variants_agree_and_autotune(ctx):
             f.flow(0.02, 3)
             outs.append(f.download())
             f.close()
-        for o in outs[1:]:
-            assert np.array_equal(o, outs[0])
+        # variants in one group compute the same links bit for bit
+        npl = lgf.FP32_SHARED_PLAQUETTE_VARIANTS if prec == lgf.FP32 else 0
+        for grp in (outs[:npl], outs[npl:]):
+            for o in grp[1:]:
+                assert np.array_equal(o, grp[0])
+        if npl:
+            assert np.max(np.abs(outs[npl] - outs[0])) < 2e-6
         best, ms = ctx.autotune(shape, prec, 2)
         assert 0 <= best < len(ms) and all(m > 0 for m in ms)
         assert ctx.stage_variant(prec) == best
