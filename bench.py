@@ -369,7 +369,9 @@ def main():
                    "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
-                     "kernel": "k_rk3_stage<float> (3 launches per RK3 step, 576 B per link-update algorithmic)"},
+                     "kernel": "%s (stage variant %d, 3 launches per RK3 step, 576 B per link-update algorithmic)" % (
+                         "k_rk3_stage_pl<float>" if r32["stage_variant"] < lgf.FP32_SHARED_PLAQUETTE_VARIANTS
+                         else "k_rk3_stage<float>", r32["stage_variant"])},
     }
     if not args.no_batch:
         line["batched_16_uneven"] = batched(lgf, ctx)
