@@ -129,13 +129,28 @@ def run_job(lgf, ctx, src, prec, steps, warmup, ev):
     E.append(e[0]), Q.append(q[0])
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    diag = f.diagnostics()
     b.close()
     f.close()
+    # untimed replay of the same trajectory (the flow is deterministic): the
+    # deviation from unitarity, max and mean of |U^dag U - 1| over the links,
+    # at every measurement time
+    f = lgf.Field(ctx, SHAPE, prec)
+    f.copy_from(src)
+    b = lgf.Batch(ctx, [f])
+    unit = [f.diagnostics()[1:]]
+    done = 0
+    while done < steps:
+        n = min(MEASURE_EVERY, steps - done)
+        b.flow(EPS, n, use_graph=True)
+        done += n
+        unit.append(f.diagnostics()[1:])
+    b.close()
+    f.close()
+    unit = np.array(unit)
     t = np.array([0.0] + [EPS * k for k in range(MEASURE_EVERY, steps + 1, MEASURE_EVERY)])[: len(E)]
     return {"wall_s": wall, "flow_ms_per_step": flow_ms, "measure_ms": meas_ms, "stage_variant": variant,
             "tile_order": ctx.tile_order(), "autotune_ms_per_step": tune_ms, "t": t, "E": np.array(E),
-            "Q": np.array(Q), "unitarity_max": diag[1]}
+            "Q": np.array(Q), "unitarity_max": float(unit[:, 0].max()), "unitarity": unit}
 
 
 def t0_of(t, E):
@@ -335,6 +350,8 @@ def main():
         log("%s: job %.3f s, flow %.4f ms/step, measure %.3f ms, variant %d, tile order %d" % (
             "fp64" if prec == 0 else "fp32", r["wall_s"], r["flow_ms_per_step"], r["measure_ms"], r["stage_variant"],
             r["tile_order"]))
+        log("%s: |U^dag U - 1| max/mean at t = 0, 0.05, ...: %s" % (
+            "fp64" if prec == 0 else "fp32", " ".join("%.2e/%.2e" % (a, m) for a, m in r["unitarity"])))
     lu_step = 4.0 * np.prod(SHAPE)
     r32, r64 = res[lgf.FP32], res[lgf.FP64]
     acc = accuracy(r64, r32)
@@ -365,6 +382,10 @@ def main():
         "measure_ms": {"fp32": r32["measure_ms"], "fp64": r64["measure_ms"]},
         "fp32_vs_fp64": acc,
         "unitarity_max": {"fp32": r32["unitarity_max"], "fp64": r64["unitarity_max"]},
+        "unitarity_every_measurement": {
+            name: {"max_of_max": float(r["unitarity"][:, 0].max()), "max_of_mean": float(r["unitarity"][:, 1].max()),
+                   "final_max": float(r["unitarity"][-1, 0]), "final_mean": float(r["unitarity"][-1, 1]),
+                   "measurements": int(len(r["unitarity"]))} for name, r in (("fp32", r32), ("fp64", r64))},
         "tuning": {"stage_variant": {"fp64": r64["stage_variant"], "fp32": r32["stage_variant"]},
                    "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",

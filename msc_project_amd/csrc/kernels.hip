@@ -349,10 +349,13 @@ LGF_DEV M3<C> line(const Lines<S, C>& L, int dir, int site) {
     else return load_link<C, C>(L.L3, L.G, dir, L.lbase + site);
 }
 
-LGF_DEV int wrap_offset(const Geo& g, int dir, int steps) {
+// site offset of x + steps*dir (|steps| <= 3, periodic); rcp = the field's
+// (1 + 2^-40)/n reciprocals, so the wrap is one double multiply (fdiv)
+// instead of an integer remainder
+LGF_DEV int wrap_offset(const Geo& g, int dir, int steps, const double* rcp) {
     const int xd = geo_x(g, dir), nd = geo_n(g, dir);
-    int v = (xd + steps) % nd;
-    v = v < 0 ? v + nd : v;
+    const int t = xd + steps + 4 * nd;  // >= 0: nd >= 2
+    const int v = t - fdiv(t, rcp[dir]) * nd;
     return (v - xd) * geo_str(g, dir);
 }
 
@@ -426,12 +429,12 @@ template <typename C, typename T>
 LGF_DEV void herm_pack(const M3<C>& F, T* h, int step);
 
 template <typename S, typename C, typename T>
-LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fimp, T* f11) {
+LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, const double* rcp, int mu, int nu, M3<C>& Fimp, T* f11) {
     PlaneOff o;
 #pragma unroll
     for (int k = -3; k <= 3; ++k) {
-        o.m[3 + k] = wrap_offset(g, mu, k);
-        o.n[3 + k] = wrap_offset(g, nu, k);
+        o.m[3 + k] = k == 0 ? 0 : wrap_offset(g, mu, k, rcp);
+        o.n[3 + k] = k == 0 ? 0 : wrap_offset(g, nu, k, rcp);
     }
     const int s = g.s;
     const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
@@ -448,23 +451,34 @@ LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, int mu, int nu, M3<C>& Fim
         e = retr_mul(F11, F11);
         herm_pack(F11, f11, kTileSites);
     }
-    scale(acc, k1);
+    // Leaves of one weight are added unscaled and the sum is rescaled between
+    // weights: acc = k1 avg11 / w22, + S22, * w22/w12, + S12, * w12/w13,
+    // + S13, * w13, with w22 = k2/4, w12 = k3/8, w13 = k4/8 the per-leaf
+    // weights of the 2x2, 1x2 and 1x3 clover averages.
+    const C w22 = C(0.25) * k2, w12 = C(0.125) * k3, w13 = C(0.125) * k4;
+    scale(acc, k1 / w22);
 #pragma unroll 1
     for (int it = 0; it < 2; ++it) {
-        diag_leaf<2, 2>(acc, C(0.25) * k2, L, s, o, mu, nu, it);
-        diag_leaf<1, 2>(acc, C(0.125) * k3, L, s, o, mu, nu, it);
-        diag_leaf<2, 1>(acc, C(0.125) * k3, L, s, o, mu, nu, it);
-        diag_leaf<1, 3>(acc, C(0.125) * k4, L, s, o, mu, nu, it);
-        diag_leaf<3, 1>(acc, C(0.125) * k4, L, s, o, mu, nu, it);
+        diag_leaf<2, 2>(acc, C(1), L, s, o, mu, nu, it);
+        side_leaf<2, 2>(acc, C(1), L, s, o, mu, nu, it);
     }
+    scale(acc, w22 / w12);
 #pragma unroll 1
     for (int it = 0; it < 2; ++it) {
-        side_leaf<2, 2>(acc, C(0.25) * k2, L, s, o, mu, nu, it);
-        side_leaf<1, 2>(acc, C(0.125) * k3, L, s, o, mu, nu, it);
-        side_leaf<2, 1>(acc, C(0.125) * k3, L, s, o, mu, nu, it);
-        side_leaf<1, 3>(acc, C(0.125) * k4, L, s, o, mu, nu, it);
-        side_leaf<3, 1>(acc, C(0.125) * k4, L, s, o, mu, nu, it);
+        diag_leaf<1, 2>(acc, C(1), L, s, o, mu, nu, it);
+        diag_leaf<2, 1>(acc, C(1), L, s, o, mu, nu, it);
+        side_leaf<1, 2>(acc, C(1), L, s, o, mu, nu, it);
+        side_leaf<2, 1>(acc, C(1), L, s, o, mu, nu, it);
     }
+    scale(acc, w12 / w13);
+#pragma unroll 1
+    for (int it = 0; it < 2; ++it) {
+        diag_leaf<1, 3>(acc, C(1), L, s, o, mu, nu, it);
+        diag_leaf<3, 1>(acc, C(1), L, s, o, mu, nu, it);
+        side_leaf<1, 3>(acc, C(1), L, s, o, mu, nu, it);
+        side_leaf<3, 1>(acc, C(1), L, s, o, mu, nu, it);
+    }
+    scale(acc, w13);
     Fimp = project_F(acc);
     return e;
 }
@@ -510,7 +524,7 @@ __global__ __launch_bounds__(256) void k_lines(const FieldDesc* __restrict__ fds
     const int li = (d.tile0 - t0) * kTileSites + s;
     store_link<C, C>(lines, G, mu, li, L2);
     store_link<C, C>(lines + 4 * 18 * G, G, mu, li,
-                     su3_nn(L2, load_link<S, C>(U, d.stride, mu, s + wrap_offset(g, mu, 2))));
+                     su3_nn(L2, load_link<S, C>(U, d.stride, mu, s + wrap_offset(g, mu, 2, d.rcp))));
 }
 
 // One thread per (site, plane).  The six planes are taken in the order of
@@ -556,7 +570,7 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(LGF_CLOVER_
         const int mu = (p & 1) ? (p == 1 ? 0 : (p == 3 ? 2 : 1)) : 3;
         const int nu = (p & 1) ? (p == 1 ? 1 : (p == 3 ? 0 : 2)) : (p == 0 ? 2 : (p == 2 ? 1 : 0));
         M3<C> Fi;
-        e = (double)plane_F<S, C, T>(L, g, mu, nu, Fi, &sF11[p][0][threadIdx.x]);
+        e = (double)plane_F<S, C, T>(L, g, d.rcp, mu, nu, Fi, &sF11[p][0][threadIdx.x]);
         ei = (double)retr_mul(Fi, Fi);
         herm_pack(Fi, &sF[p][0][threadIdx.x], kTileSites);
     } else {
