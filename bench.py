@@ -240,12 +240,18 @@ def serving(lgf, nreq, throughput, seed=4242, policy=1, age_rate=0.0):
                 time.sleep(dt)
             ids[i] = srv.submit(shape, EPS, n, 10, precision=lgf.FP32, seed=sd, beta=beta,
                                 measure_flags=lgf.MEASURE_DEFAULT)
+            if nreq >= 100 and (i + 1) % 50 == 0:
+                log("serving: %d of %d requests submitted" % (i + 1, nreq))
 
     t0 = time.perf_counter()
     th = threading.Thread(target=feeder)
     th.start()
     th.join()
-    res = [srv.wait(i, timeout=600) for i in ids]
+    res = []
+    for k, i in enumerate(ids):
+        res.append(srv.wait(i, timeout=600))
+        if nreq >= 100 and (k + 1) % 50 == 0:
+            log("serving: %d of %d requests done" % (k + 1, nreq))
     wall = time.perf_counter() - t0
     st = srv.stats()
     srv.close()
