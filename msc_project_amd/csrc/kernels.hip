@@ -145,13 +145,59 @@ struct PlaqLds {
     C v[6][9][kTileSites];  // [plane][ta component][lane]: lane-contiguous rows, no bank conflicts
 };
 
+// The four links U_d(x) of a tile's sites in LDS (float4 planes of
+// components 0..15 and a float2 plane of 16, 17, as in global memory).  With
+// LOC each direction wave loads its own link once and leaves it here; the
+// plaquettes read the rows they need (0 and 1 of the left factor of each
+// half-plaquette) and the wave rereads its whole link for the update.
+struct LocLds {
+    vf4 q[4][4][kTileSites];
+    vf2 r[4][kTileSites];
+};
+
+template <bool ON>
+struct LocSlot {
+    LocLds b;
+    LGF_DEV LocLds* ptr() { return &b; }
+};
+template <>
+struct LocSlot<false> {
+    LGF_DEV LocLds* ptr() { return nullptr; }
+};
+
+LGF_DEV void loc_store(LocLds& l, int d, int lane, const M3<float>& m) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) l.q[d][k][lane] = vf4{m.v[4 * k], m.v[4 * k + 1], m.v[4 * k + 2], m.v[4 * k + 3]};
+    l.r[d][lane] = vf2{m.v[16], m.v[17]};
+}
+
+LGF_DEV M3<float> loc_load(const LocLds& l, int d, int lane) {
+    M3<float> m;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const vf4 v = l.q[d][k][lane];
+        m.v[4 * k] = v.x, m.v[4 * k + 1] = v.y, m.v[4 * k + 2] = v.z, m.v[4 * k + 3] = v.w;
+    }
+    const vf2 w = l.r[d][lane];
+    m.v[16] = w.x, m.v[17] = w.y;
+    return m;
+}
+
+// U_d(x) from global memory or, with LOC, from LDS
+template <bool LOC, typename S, typename C>
+LGF_DEV M3<C> local_link(const S* __restrict__ U, int64_t st, int d, int s, const LocLds* loc, int lane) {
+    if constexpr (LOC) return loc_load(*loc, d, lane);
+    else return load_link<S, C>(U, st, d, s);
+}
+
 // -1/2 TA(P_ab(x)) of one plane into LDS; returns the last value computed
-template <typename S, typename C, int A, int B>
-LGF_DEV C plaquette_to_lds(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, const Geo& g, int lane) {
+template <typename S, typename C, int A, int B, bool LOC>
+LGF_DEV C plaquette_to_lds(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, const Geo& g, int lane,
+                           const LocLds* loc) {
     const int s = g.s;
     const int fa = nb_off(g, A, +1), fb = nb_off(g, B, +1);
-    const M3<C> h1 = su3_nn(load_link<S, C>(U, st, A, s), load_link<S, C>(U, st, B, s + fa));
-    const M3<C> h2 = su3_nn(load_link<S, C>(U, st, B, s), load_link<S, C>(U, st, A, s + fb));
+    const M3<C> h1 = su3_nn(local_link<LOC, S, C>(U, st, A, s, loc, lane), load_link<S, C>(U, st, B, s + fa));
+    const M3<C> h2 = su3_nn(local_link<LOC, S, C>(U, st, B, s, loc, lane), load_link<S, C>(U, st, A, s + fb));
     C t[9];
     ta_pack(ta_of_product_nd(h1, h2, C(-0.5)), t);
     constexpr int p = plane_index(A, B);
@@ -174,30 +220,31 @@ LGF_DEV void after(Geo& g, C dep) {
 
 // this wave's share of the site's six plaquettes; returns the last value
 // computed
-template <typename S, typename C, int MU, bool ORDERED>
-LGF_DEV C plaquettes_of(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, Geo g, int lane) {
+template <typename S, typename C, int MU, bool ORDERED, bool LOC>
+LGF_DEV C plaquettes_of(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, Geo g, int lane, const LocLds* loc) {
     C dep;
     if constexpr (MU == 0) {
-        dep = plaquette_to_lds<S, C, 0, 1>(sp, U, st, g, lane);
+        dep = plaquette_to_lds<S, C, 0, 1, LOC>(sp, U, st, g, lane, loc);
         after<ORDERED>(g, dep);
-        dep = plaquette_to_lds<S, C, 0, 2>(sp, U, st, g, lane);
+        dep = plaquette_to_lds<S, C, 0, 2, LOC>(sp, U, st, g, lane, loc);
     } else if constexpr (MU == 1) {
-        dep = plaquette_to_lds<S, C, 1, 2>(sp, U, st, g, lane);
+        dep = plaquette_to_lds<S, C, 1, 2, LOC>(sp, U, st, g, lane, loc);
         after<ORDERED>(g, dep);
-        dep = plaquette_to_lds<S, C, 1, 3>(sp, U, st, g, lane);
+        dep = plaquette_to_lds<S, C, 1, 3, LOC>(sp, U, st, g, lane, loc);
     } else if constexpr (MU == 2) {
-        dep = plaquette_to_lds<S, C, 2, 3>(sp, U, st, g, lane);
+        dep = plaquette_to_lds<S, C, 2, 3, LOC>(sp, U, st, g, lane, loc);
     } else {
-        dep = plaquette_to_lds<S, C, 0, 3>(sp, U, st, g, lane);
+        dep = plaquette_to_lds<S, C, 0, 3, LOC>(sp, U, st, g, lane, loc);
     }
     return dep;
 }
 
 // Everything of link (x,MU) before the barrier: this wave's plaquettes and
 // -1/2 TA(U_mu sum_nu lower_nu) in the nine-real form.
-template <typename S, typename C, int MU, bool ORDERED>
-LGF_DEV void stage_pre_pl(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, Geo g, int lane, C* z, M3<C>& Umu) {
-    const C dep = plaquettes_of<S, C, MU, ORDERED>(sp, U, st, g, lane);
+template <typename S, typename C, int MU, bool ORDERED, bool LOC>
+LGF_DEV void stage_pre_pl(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, Geo g, int lane, C* z, M3<C>& Umu,
+                          const LocLds* loc) {
+    const C dep = plaquettes_of<S, C, MU, ORDERED, LOC>(sp, U, st, g, lane, loc);
     after<ORDERED>(g, dep);
     constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
     const int fmu = nb_off(g, MU, +1);
@@ -208,7 +255,7 @@ LGF_DEV void stage_pre_pl(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, G
     lower_term<S, C, MU, N1>(A, U, st, g, fmu);
     after<ORDERED>(g, A.v[17]);
     lower_term<S, C, MU, N2>(A, U, st, g, fmu);
-    Umu = load_link<S, C>(U, st, MU, g.s);
+    Umu = local_link<LOC, S, C>(U, st, MU, g.s, loc, lane);
     ta_pack(ta_of_product(Umu, A, C(-0.5)), z);
 }
 
@@ -226,10 +273,13 @@ LGF_DEV void add_planes(const PlaqLds<C>& sp, int mu, int lane, C* z) {
     }
 }
 
-template <typename S, typename C, int STAGE, int WPE, bool ORDERED>
+template <typename S, typename C, int STAGE, int WPE, bool ORDERED, bool LOC>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage_pl(
     const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int ping) {
+    static_assert(std::is_same<S, float>::value && std::is_same<C, float>::value, "mixed-precision path only");
     __shared__ PlaqLds<C> sp;
+    __shared__ LocSlot<LOC> lslot;
+    LocLds* loc = lslot.ptr();
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int2 tm = tiles[t];
     const FieldDesc& d = fds[tm.x];
@@ -246,13 +296,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     const Geo g = make_geo(d, s);
     const C eps = (C)d.eps;
 
+    if constexpr (LOC) {
+        loc_store(*loc, mu, lane, load_link<S, C>(U, st, mu, s));
+        __syncthreads();
+    }
     C z[9];
     M3<C> Umu;
     switch (mu) {
-        case 0: stage_pre_pl<S, C, 0, ORDERED>(sp, U, st, g, lane, z, Umu); break;
-        case 1: stage_pre_pl<S, C, 1, ORDERED>(sp, U, st, g, lane, z, Umu); break;
-        case 2: stage_pre_pl<S, C, 2, ORDERED>(sp, U, st, g, lane, z, Umu); break;
-        default: stage_pre_pl<S, C, 3, ORDERED>(sp, U, st, g, lane, z, Umu); break;
+        case 0: stage_pre_pl<S, C, 0, ORDERED, LOC>(sp, U, st, g, lane, z, Umu, loc); break;
+        case 1: stage_pre_pl<S, C, 1, ORDERED, LOC>(sp, U, st, g, lane, z, Umu, loc); break;
+        case 2: stage_pre_pl<S, C, 2, ORDERED, LOC>(sp, U, st, g, lane, z, Umu, loc); break;
+        default: stage_pre_pl<S, C, 3, ORDERED, LOC>(sp, U, st, g, lane, z, Umu, loc); break;
     }
     __syncthreads();
     add_planes(sp, mu, lane, z);
@@ -804,24 +858,27 @@ static hipError_t stage_launch(int stage, const FieldDesc* fds, const int2* tile
     return hipGetLastError();
 }
 
-template <int WPE, bool ORDERED>
+template <int WPE, bool ORDERED, bool LOC = false>
 static hipError_t stage_launch_pl(int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
                                   hipStream_t stream) {
     dim3 grid(ntiles), block(kTileSites, 4);
     if (stage == 0)
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 0, WPE, ORDERED>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 0, WPE, ORDERED, LOC>), grid, block, 0, stream, fds, tiles,
+                           ping);
     else if (stage == 1)
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 1, WPE, ORDERED>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 1, WPE, ORDERED, LOC>), grid, block, 0, stream, fds, tiles,
+                           ping);
     else
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 2, WPE, ORDERED>), grid, block, 0, stream, fds, tiles, ping);
+        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 2, WPE, ORDERED, LOC>), grid, block, 0, stream, fds, tiles,
+                           ping);
     return hipGetLastError();
 }
 
 // Stage-kernel variants: (register budget as minimum waves per SIMD, launch
-// shape).  Mixed-precision variants 0..2 are the shared-plaquette kernel,
-// 3..8 the per-link kernel.  Variant 0 is the default (a budget the kernel
+// shape).  Mixed-precision variants 0..4 are the shared-plaquette kernel
+// (0 and 1 with the site's links in LDS), 5..10 the per-link kernel.  Variant 0 is the default (a budget the kernel
 // fits without spilling); lgf_autotune picks the fastest.
-int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 9; }
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 11; }
 
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream) {
@@ -835,15 +892,17 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
         }
     }
     switch (variant) {
-        case 1: return stage_launch_pl<4, false>(stage, fds, tiles, ntiles, ping, stream);
-        case 2: return stage_launch_pl<5, true>(stage, fds, tiles, ntiles, ping, stream);
-        case 3: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 4: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 5: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 6: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 7: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
-        case 8: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
-        default: return stage_launch_pl<4, true>(stage, fds, tiles, ntiles, ping, stream);
+        case 1: return stage_launch_pl<4, false, true>(stage, fds, tiles, ntiles, ping, stream);
+        case 2: return stage_launch_pl<4, true>(stage, fds, tiles, ntiles, ping, stream);
+        case 3: return stage_launch_pl<4, false>(stage, fds, tiles, ntiles, ping, stream);
+        case 4: return stage_launch_pl<5, true>(stage, fds, tiles, ntiles, ping, stream);
+        case 5: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 6: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 7: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 8: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 9: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
+        case 10: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
+        default: return stage_launch_pl<4, true, true>(stage, fds, tiles, ntiles, ping, stream);
     }
 }
 
