@@ -131,10 +131,11 @@ def test_config1_golden_fp64(ctx, config1):
         assert q_err(o["Q_unimproved"], c["measured_Q_unimproved"], o["Qabs"]) < TOL_FP64
 
 
-def test_observables_match_oracle(ctx):
+@pytest.mark.parametrize("shape", [(6, 4, 4, 8), (6, 4, 2, 2), (2, 2, 4, 6)])
+def test_observables_match_oracle(ctx, shape):
     """improved E and unimproved Q (MeasureFlowedGaugeConfigurations) against
-    the oracle, in every precision mode"""
-    shape = (6, 4, 4, 8)
+    the oracle, in every precision mode; 6x4x2x2 and 2x2x4x6 have a ragged
+    last tile (96 sites), 2x2x4x6 lines that wrap onto their own start"""
     U = O.generate(shape, 6.0, seed=61, start="hot", sweeps=3)
     ref = {"E": O.energy_density(shape, U), "Q": O.top_charge(shape, U)[0],
            "E_improved": O.energy_density(shape, U, improved=True), "Q_unimproved": O.top_charge_unimproved(shape, U)}
