@@ -684,15 +684,19 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(LGF_CLOVER_
 // field's result does not depend on the other fields of its batch.
 // out[kOutStride*f + c]; for the clover partials c = {sum e, sum q, sum |q|,
 // sum ei, sum q11}.
-__global__ __launch_bounds__(256) void k_reduce(const FieldDesc* __restrict__ fds, const double* __restrict__ partials,
-                                                int pstride, int ncomp, double* __restrict__ out) {
-    __shared__ double red[8][4];
+constexpr int kReduceThreads = 1024;  // 16 waves: the tile loop is latency bound
+
+__global__ __launch_bounds__(kReduceThreads) void k_reduce(const FieldDesc* __restrict__ fds,
+                                                           const double* __restrict__ partials, int pstride,
+                                                           int ncomp, double* __restrict__ out) {
+    constexpr int kWaves = kReduceThreads / 64;
+    __shared__ double red[8][kWaves];
     const FieldDesc& d = fds[blockIdx.x];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     double acc[8];
 #pragma unroll
     for (int c = 0; c < 8; ++c) acc[c] = 0.0;
-    for (int t = threadIdx.x; t < d.ntiles; t += 256) {
+    for (int t = threadIdx.x; t < d.ntiles; t += kReduceThreads) {
         const double* rec = partials + (int64_t)pstride * (d.tile0 + t);
 #pragma unroll
         for (int c = 0; c < 8; ++c)
@@ -707,7 +711,15 @@ __global__ __launch_bounds__(256) void k_reduce(const FieldDesc* __restrict__ fd
     __syncthreads();
     if ((int)threadIdx.x < ncomp) {
         const int c = threadIdx.x;
-        out[kOutStride * blockIdx.x + c] = (red[c][0] + red[c][1]) + (red[c][2] + red[c][3]);
+        double v[kWaves];
+#pragma unroll
+        for (int k = 0; k < kWaves; ++k) v[k] = red[c][k];
+        // pairwise, in a fixed order
+#pragma unroll
+        for (int h = kWaves / 2; h > 0; h >>= 1)
+#pragma unroll
+            for (int k = 0; k < h; ++k) v[k] += v[k + h];
+        out[kOutStride * blockIdx.x + c] = v[0];
     }
 }
 
@@ -936,7 +948,7 @@ hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2
 
 hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int pstride, int ncomp, double* out,
                          hipStream_t stream) {
-    hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(256), 0, stream, fds, partials, pstride, ncomp, out);
+    hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(kReduceThreads), 0, stream, fds, partials, pstride, ncomp, out);
     return hipGetLastError();
 }
 
@@ -946,7 +958,7 @@ hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int nt
         hipLaunchKernelGGL((k_diag<double>), dim3(ntiles), dim3(64), 0, stream, fds, tiles, ping, partials);
     else
         hipLaunchKernelGGL((k_diag<float>), dim3(ntiles), dim3(64), 0, stream, fds, tiles, ping, partials);
-    hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(256), 0, stream, fds, partials, 3, 2, out);
+    hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(kReduceThreads), 0, stream, fds, partials, 3, 2, out);
     hipLaunchKernelGGL(k_reduce_max, dim3(nfields), dim3(256), 0, stream, fds, partials, 3, 2, out);
     return hipGetLastError();
 }
