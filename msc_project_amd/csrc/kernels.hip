@@ -11,6 +11,9 @@
 //              eps X1 and eps (3/4 Z2 - X1), which is the same integrator
 //              with one accumulator (also the form of the thesis algorithm,
 //              doc/thesis.tex:2569-2600).
+// k_rk3_stage_pl  the same stage for the mixed-precision path with each
+//              site's six plaquettes computed once for its four links (the
+//              four direction waves of a workgroup share them in LDS).
 // k_clover     1x1 clover F (energy density) and the improved F (19/9 1x1 +
 //              1/36 2x2 - 32/45 1x2 + 1/15 1x3, LGF/Field/SU3_field.cpp:533-797)
 //              for the plane pairs of the topological charge; per-site q(x),
