@@ -279,13 +279,14 @@ def test_bad_shape_rejected(ctx):
     assert e.value.code == 2
 
 
+@pytest.mark.parametrize("shape", [(4, 4, 6, 4), (2, 4, 2, 6), (6, 2, 4, 2)])
 @pytest.mark.parametrize("variant", [0, lgf.FP32_SHARED_PLAQUETTE_VARIANTS])
-def test_fp32_flow_tracks_oracle_on_hot_start(ctx, variant):
+def test_fp32_flow_tracks_oracle_on_hot_start(ctx, variant, shape):
     """the mixed-precision flow (7-term series exponential, FP64 projection
     per step) stays within FP32 rounding of the FP64 oracle on a rough hot
     start, where the exponents are largest, also at eps = 0.05 where the
-    scaling-and-squaring branch is taken"""
-    shape = (4, 4, 6, 4)
+    scaling-and-squaring branch is taken; lattices with extent 2 (neighbours
+    x+mu and x-mu coincide) and ragged last tiles (96 sites) included"""
     U = O.generate(shape, 6.0, seed=91, start="hot", sweeps=0)
     ctx.set_stage_variant(lgf.FP32, variant)
     for eps in (0.01, 0.05):
