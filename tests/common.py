@@ -53,3 +53,45 @@ def q_err(Q, Qref, Qabs):
 def haar_field(shape, seed):
     from oracle import oracle as O
     return O.generate(shape, 6.0, seed=seed, start="hot", sweeps=0)
+
+
+def flux_profile(a):
+    """improved field strength 19/9 F11 + 1/36 F22 - 32/45 F12 + 1/15 F13 of a
+    uniform abelian plaquette phase a (each m x n leaf has phase m n a)"""
+    return 19 / 9 * np.sin(a) + 1 / 36 * np.sin(4 * a) - 32 / 45 * np.sin(2 * a) + 1 / 15 * np.sin(3 * a)
+
+
+def twisted_flux_field(shape, n01, n23):
+    """Abelian field diag(e^{i th}, e^{-i th}, 1) with n01 flux quanta through
+    every (0,1) plane and n23 through every (2,3) plane: U_1(x) = e^{i p x0},
+    U_0(x) = e^{-i p L0 x1} on the slice x0 = L0 - 1 and 1 elsewhere,
+    p = 2 pi n01 / (L0 L1); likewise U_3, U_2 with (x2, x3) and q.  Every
+    plaquette of those planes is e^{i p} (e^{i q}), every contractible m x n
+    rectangle e^{i m n p}, the mixed planes are trivial.  Returns (U, p, q)
+    in file order."""
+    L0, L1, L2, L3 = shape
+    x = np.indices(shape).reshape(4, -1)
+    p = 2 * np.pi * n01 / (L0 * L1)
+    q = 2 * np.pi * n23 / (L2 * L3)
+    th = np.zeros((x.shape[1], 4))
+    th[:, 1] = p * x[0]
+    th[:, 0] = np.where(x[0] == L0 - 1, -p * L0 * x[1], 0.0)
+    th[:, 3] = q * x[2]
+    th[:, 2] = np.where(x[2] == L2 - 1, -q * L2 * x[3], 0.0)
+    U = np.zeros((x.shape[1], 4, 18))
+    U[:, :, 0], U[:, :, 1] = np.cos(th), np.sin(th)
+    U[:, :, 8], U[:, :, 9] = np.cos(th), -np.sin(th)
+    U[:, :, 16] = 1.0
+    return U, p, q
+
+
+def twisted_flux_observables(shape, p, q):
+    """closed forms of the measured observables of twisted_flux_field: E and
+    the improved E sum Re Tr(F F) over the two planes (2 sin^2 per plane),
+    the charges are V/(4 pi^2) times the per-site -2 F01 F23"""
+    V = float(np.prod(shape))
+    return {"E": 2 * np.sin(p) ** 2 + 2 * np.sin(q) ** 2,
+            "E_improved": 2 * flux_profile(p) ** 2 + 2 * flux_profile(q) ** 2,
+            "Q": -2 * V * flux_profile(p) * flux_profile(q) / (4 * np.pi ** 2),
+            "Q_unimproved": -2 * V * np.sin(p) * np.sin(q) / (4 * np.pi ** 2),
+            "qraw": -2 * flux_profile(p) * flux_profile(q)}

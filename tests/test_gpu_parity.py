@@ -11,7 +11,8 @@ import pytest
 
 from msc_project_amd import lgf
 from oracle import oracle as O
-from tests.common import TOL_FP64, config1_inputs, golden, q_err, rel_err, small_input
+from tests.common import (TOL_FP64, config1_inputs, golden, q_err, rel_err, small_input, twisted_flux_field,
+                          twisted_flux_observables)
 
 pytestmark = pytest.mark.gpu
 
@@ -129,6 +130,29 @@ def test_config1_golden_fp64(ctx, config1):
         o = f.observables()
         assert rel_err(o["E_improved"], c["measured_E_improved"]) < TOL_FP64
         assert q_err(o["Q_unimproved"], c["measured_Q_unimproved"], o["Qabs"]) < TOL_FP64
+
+
+@pytest.mark.parametrize("shape,n01,n23", [((8, 8, 6, 6), 1, 1), ((6, 4, 4, 6), 1, -2), ((12, 12, 8, 8), 2, 1)])
+def test_twisted_flux_closed_form(ctx, shape, n01, n23):
+    """the clover kernels against what the mathematics fixes: integer flux
+    through the (0,1) and (2,3) tori gives closed forms for E, the improved E,
+    Q, the unimproved Q and q(x), and an improved charge within O(p^4) of
+    the integer -2 n01 n23; in every precision mode"""
+    U, p, q = twisted_flux_field(shape, n01, n23)
+    ref = twisted_flux_observables(shape, p, q)
+    target = -2 * n01 * n23
+    for prec, flags, tol in ((lgf.FP64, lgf.MEASURE_DEFAULT, 1e-12), (lgf.FP32, lgf.MEASURE_FP64, 2e-5),
+                             (lgf.FP32, lgf.MEASURE_DEFAULT, 2e-5)):
+        f = lgf.Field(ctx, shape, prec)
+        f.upload(U)
+        o = f.observables(flags, want_q=True)
+        for k in ("E", "E_improved"):
+            assert rel_err(o[k], ref[k]) < tol, (prec, flags, k)
+        for k in ("Q", "Q_unimproved"):
+            assert abs(o[k] - ref[k]) < tol * abs(target), (prec, flags, k)
+        assert np.max(np.abs(o["qraw"] - ref["qraw"])) < tol * abs(ref["qraw"]), (prec, flags)
+        assert abs(o["Q"] - target) < 2 * abs(target) * max(abs(p), abs(q)) ** 4 + tol * abs(target)
+        f.close()
 
 
 @pytest.mark.parametrize("shape", [(6, 4, 4, 8), (6, 4, 2, 2), (2, 2, 4, 6)])

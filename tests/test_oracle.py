@@ -19,7 +19,8 @@ import pytest
 import scipy.linalg
 
 from oracle import oracle as O
-from tests.common import TOL_FP64, golden, q_err, rel_err, small_input, config1_inputs
+from tests.common import (TOL_FP64, config1_inputs, golden, q_err, rel_err, small_input, twisted_flux_field,
+                          twisted_flux_observables)
 
 RNG = np.random.default_rng(7)
 
@@ -277,6 +278,25 @@ def test_clover_abelian_closed_form(shape, kb, kc):
     assert np.max(np.abs(F - np.diag([-np.sin(b), np.sin(b), 0]))) < 1e-15
     F = to_c(O.field_strength(shape, U, 3, 0, 1, 4))
     assert np.max(np.abs(F - improved_profile(b) * np.diag([-1, 1, 0]))) < 1e-14
+
+
+@pytest.mark.parametrize("shape,n01,n23", [((8, 8, 6, 6), 1, 1), ((6, 4, 4, 6), 1, -2), ((12, 12, 8, 8), 2, 1)])
+def test_twisted_flux_closed_form_and_integer_charge(shape, n01, n23):
+    """integer flux through the (0,1) and (2,3) tori: the observables follow
+    their closed forms, and the improved charge approaches the integer
+    -2 n01 n23 with O(p^4) corrections (the improvement cancels O(p^2))"""
+    U, p, q = twisted_flux_field(shape, n01, n23)
+    ref = twisted_flux_observables(shape, p, q)
+    # the twisted links carry phases up to 2 pi n: rounding of ~1e-15 per link
+    assert rel_err(O.energy_density(shape, U), ref["E"]) < 1e-12
+    assert rel_err(O.energy_density(shape, U, improved=True), ref["E_improved"]) < 1e-12
+    Q, _, qx = O.top_charge(shape, U, want_qx=True)
+    assert abs(Q - ref["Q"]) < 1e-12
+    assert np.max(np.abs(qx - np.pi ** 2 / 4 * ref["qraw"])) < 1e-13
+    assert abs(O.top_charge_unimproved(shape, U) - ref["Q_unimproved"]) < 1e-12
+    target = -2 * n01 * n23
+    assert abs(ref["Q"] - target) < 2 * abs(target) * (max(abs(p), abs(q)) ** 4)
+    assert abs(ref["Q"] - target) < abs(ref["Q_unimproved"] - target)
 
 
 def test_clover_cold_zero():
