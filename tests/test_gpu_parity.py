@@ -155,6 +155,25 @@ def test_twisted_flux_closed_form(ctx, shape, n01, n23):
         f.close()
 
 
+def test_flow_keeps_constant_abelian_field(ctx):
+    """a constant abelian field strength is a stationary point of the flow
+    (the upper and lower plaquettes of every link cancel in the projection):
+    the links stay where they are, in both precisions and every stage variant
+    group"""
+    shape = (6, 4, 4, 6)
+    U, p, q = twisted_flux_field(shape, 1, -2)
+    for prec, tol in ((lgf.FP64, 1e-13), (lgf.FP32, 2e-6)):
+        variants = [0, lgf.FP32_SHARED_PLAQUETTE_VARIANTS] if prec == lgf.FP32 else [0]
+        for v in variants:
+            ctx.set_stage_variant(prec, v)
+            f = lgf.Field(ctx, shape, prec)
+            f.upload(U)
+            f.flow(0.05, 3)
+            assert np.max(np.abs(f.download() - U)) < tol, (prec, v)
+            f.close()
+        ctx.set_stage_variant(prec, 0)
+
+
 @pytest.mark.parametrize("shape", [(6, 4, 4, 8), (6, 4, 2, 2), (2, 2, 4, 6)])
 def test_observables_match_oracle(ctx, shape):
     """improved E and unimproved Q (MeasureFlowedGaugeConfigurations) against

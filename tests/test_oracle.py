@@ -299,6 +299,19 @@ def test_twisted_flux_closed_form_and_integer_charge(shape, n01, n23):
     assert abs(ref["Q"] - target) < abs(ref["Q_unimproved"] - target)
 
 
+def test_flow_keeps_constant_abelian_field():
+    """a constant abelian field strength is a stationary point of the Wilson
+    flow: the upper (phase p) and lower (phase -p) plaquettes of every link
+    cancel in the projection, so Z = 0 and the links do not move; a wrong
+    sign or neighbour in the staple sum breaks this"""
+    shape = (6, 4, 4, 6)
+    U, p, q = twisted_flux_field(shape, 1, -2)
+    V = U.copy()
+    for _ in range(3):
+        O.rk3_step(shape, V, 0.05)
+    assert np.max(np.abs(V - U)) < 1e-13
+
+
 def test_clover_cold_zero():
     shape = (4, 4, 4, 4)
     U = cold(shape)
