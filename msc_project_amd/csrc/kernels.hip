@@ -360,10 +360,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 #define LGF_CLOVER_FENCE 0
 #endif
 #ifndef LGF_CLOVER_WPE
-#define LGF_CLOVER_WPE 2
+#define LGF_CLOVER_WPE 4
 #endif
 #ifndef LGF_CLOVER_LDS_REAL
-#define LGF_CLOVER_LDS_REAL 0
+#define LGF_CLOVER_LDS_REAL 1
+#endif
+//   LGF_CLOVER_ORDERED  1: a leaf's addresses (and so its loads) depend on the
+//                       previous leaf's result, so one leaf's lines are in
+//                       flight at a time (fewer registers, more waves)
+#ifndef LGF_CLOVER_ORDERED
+#define LGF_CLOVER_ORDERED 1
 #endif
 
 // Line products.  L^d_l(y) = U_d(y) U_d(y+d) ... U_d(y+(l-1)d), the product
@@ -434,6 +440,9 @@ LGF_DEV void diag_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const Plane
 #if LGF_CLOVER_FENCE
     asm volatile("" ::: "memory");  // keep the next leaf's loads from being hoisted into this one
 #endif
+#if LGF_CLOVER_ORDERED
+    asm volatile("" : "+v"(s) : "v"(acc.v[17]));
+#endif
     const int mb = o.m[3 - M], nb = o.n[3 - N], mf = o.m[3 + M], nf = o.n[3 + N];
     const int sc = it ? s + nb + mb : s;
     const int scn = it ? s + mb : s + nf;  // c + n nu
@@ -452,6 +461,9 @@ template <int L1, int L2, typename S, typename C>
 LGF_DEV void side_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
 #if LGF_CLOVER_FENCE
     asm volatile("" ::: "memory");
+#endif
+#if LGF_CLOVER_ORDERED
+    asm volatile("" : "+v"(s) : "v"(acc.v[17]));
 #endif
     const int a = it ? nu : mu, b = it ? mu : nu;
     const int m2 = o.m[3 - L2], n2 = o.n[3 - L2], m1 = o.m[3 + L1], n1 = o.n[3 + L1];
