@@ -41,6 +41,7 @@
 #include <vector>
 
 #include "../../include/lgf_flow.h"
+#include "kernels.hpp"
 
 namespace {
 
@@ -401,7 +402,11 @@ extern "C" int lgf_server_submit(lgf_server* s, const lgf_request* req, uint64_t
     if (rc) return rc;
     if (req->nsteps < 0 || req->measure_every < 0 || !(req->eps == req->eps)) return LGF_E_ARG;
     const bool fp64_lines = req->precision == LGF_FP64 || (req->measure_flags & LGF_MEASURE_FP64);
-    const size_t line_bytes = size_t(volume_of(req->shape)) * 2 * 4 * 18 * (fp64_lines ? 8 : 4);
+    // the line products of one measurement: whole 64-site tiles, as the
+    // measurement allocates them
+    const size_t tiles = (size_t(volume_of(req->shape)) + lgf::kTileSites - 1) / lgf::kTileSites;
+    const size_t line_bytes =
+        tiles * lgf::kTileSites * lgf::line_bytes_per_site(fp64_lines ? lgf::kPrecFP64 : lgf::kPrecFP32);
     if (bytes + measure_reserve(line_bytes) > s->budget) {
         std::lock_guard<std::mutex> lk(s->mu);
         s->err = "request needs " + std::to_string(bytes) + " bytes of device memory and " +
