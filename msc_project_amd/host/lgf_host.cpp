@@ -30,6 +30,12 @@ void set_measure_flags(int f) { g_measure_flags = f; }
 int measure_flags() { return g_measure_flags; }
 void set_input_method(const std::string& m) { g_input_method = m; }
 std::string input_method() { return g_input_method; }
+static int g_checkpoint_every = 0;
+void set_checkpoint_every(int n) {
+    if (n < 0) throw Error(LGF_E_ARG, "checkpoint interval must be >= 0");
+    g_checkpoint_every = n;
+}
+int checkpoint_every() { return g_checkpoint_every; }
 
 void check(int rc, const char* what) {
     if (rc != LGF_OK) {
@@ -316,6 +322,33 @@ void QdensityDistribution::calculate(double) {
 
 // ---------------------------------------------------------------------------
 // LGF/LGF.cpp:163-196
+// Flows `steps` steps; with checkpoints (lgf_host::checkpoint_every) the
+// recorded observables are appended to the CSVs and the configuration saved
+// under its flow time every n steps before the last.
+static void flow_with_checkpoints(SimulationParameters& params, FlowParameters& flowParams, GradientFlow& flowing,
+                                  SU3_field& U, TopologicalCharge& topCharge, EnergyDensity& Edensity,
+                                  const std::string& ensembleNum) {
+    const int n = lgf_host::checkpoint_every();
+    const int steps = flowParams.getflowSteps();
+    // a resumed flow measures every k steps from its pickup (LGF/LGF.cpp:197-226),
+    // so checkpoints on multiples of k keep the measurement times of one run
+    if (n > 0 && n % flowParams.getMeasuringInterval() != 0)
+        throw lgf_host::Error(LGF_E_ARG, "checkpoint interval must be a multiple of the measurement interval");
+    for (int s = 1; s <= steps; s++) {
+        flowing.flow();
+        if (n > 0 && s % n == 0 && s < steps) {
+            topCharge.saveTopologicalChargeToFile(params.getbeta(), flowing.getupdateMethod(),
+                                                  flowParams.getdataFolder(), "_" + ensembleNum);
+            Edensity.saveEnergyDensityToFile(params.getbeta(), flowing.getupdateMethod(), flowParams.getdataFolder(),
+                                             "_" + ensembleNum);
+            topCharge.clearResults();
+            Edensity.clearResults();
+            U.saveSU3ToFile(params.getbeta(), flowing.getupdateMethod(), flowParams.getdataFolder(),
+                            ensembleNum + "_Flowtime" + std::to_string(flowing.GetFlowTime()));
+        }
+    }
+}
+
 void FlowSavedGaugeConfigurations(SimulationParameters& params, FlowParameters& flowParams) {
     const std::string UpdateMethod = input_method();
     Lattice lattice(params, true);
@@ -331,7 +364,7 @@ void FlowSavedGaugeConfigurations(SimulationParameters& params, FlowParameters& 
         EnergyDensity Edensity(U);
         flowing.Include_EnergyDensity(Edensity);
         Edensity.calculate(0.0);
-        for (int s = 0; s < flowParams.getflowSteps(); s++) flowing.flow();
+        flow_with_checkpoints(params, flowParams, flowing, U, topCharge, Edensity, ensembleNum);
         double flowTime = flowing.GetFlowTime();
         topCharge.saveTopologicalChargeToFile(params.getbeta(), flowing.getupdateMethod(), flowParams.getdataFolder(),
                                               "_" + ensembleNum);
@@ -414,7 +447,7 @@ void ResumeFlowedConfiguration(SimulationParameters& params, FlowParameters& flo
         flowing.Include_TopCharge(topCharge);
         EnergyDensity Edensity(U);
         flowing.Include_EnergyDensity(Edensity);
-        for (int s = 0; s < flowParams.getflowSteps(); s++) flowing.flow();
+        flow_with_checkpoints(params, flowParams, flowing, U, topCharge, Edensity, ensembleNum);
         topCharge.saveTopologicalChargeToFile(params.getbeta(), flowing.getupdateMethod(), flowParams.getdataFolder(),
                                               "_" + ensembleNum);
         Edensity.saveEnergyDensityToFile(params.getbeta(), flowing.getupdateMethod(), flowParams.getdataFolder(),

@@ -188,6 +188,11 @@ public:
     void record(double flowTime, double value);
     void saveEnergyDensityToFile(double beta, const std::string& updateMethod, const std::string& dataFolder,
                                  const std::string& identifier = "");
+    // drop the recorded series (after a checkpoint appended it to the file)
+    void clearResults() {
+        m_resultVector.clear();
+        m_FlowMeasurementTimeVector.clear();
+    }
     const std::vector<double>& values() const { return m_resultVector; }
     const std::vector<double>& times() const { return m_FlowMeasurementTimeVector; }
 
@@ -204,6 +209,11 @@ public:
     void record(double flowTime, double value);
     void saveTopologicalChargeToFile(double beta, const std::string& updateMethod, const std::string& dataFolder,
                                      const std::string& identifier = "");
+    // drop the recorded series (after a checkpoint appended it to the file)
+    void clearResults() {
+        m_resultVector.clear();
+        m_FlowMeasurementTimeVector.clear();
+    }
     const std::vector<double>& values() const { return m_resultVector; }
     const std::vector<double>& times() const { return m_FlowMeasurementTimeVector; }
 
@@ -269,6 +279,14 @@ namespace lgf_host {
 // (the reference hard-codes "LHMC_3_of_0_330000", LGF/LGF.cpp:164)
 void set_input_method(const std::string& m);
 std::string input_method();
+// Mid-flow checkpoints of FlowSavedGaugeConfigurations and
+// ResumeFlowedConfiguration (0 = none): every n steps the observables
+// recorded so far are appended to their CSVs and the configuration is saved
+// as <id>_Flowtime<t>, the file ResumeFlowedConfiguration(..., t) reads.  A
+// run stopped after a checkpoint resumes from it and ends with the same
+// files as an uninterrupted run (the CSVs are append-only).
+void set_checkpoint_every(int n);
+int checkpoint_every();
 // <folder>ensembles/su3/beta<b>/<N0>X..X<N3>/<method>/torus_extdof4_<id>.bin
 std::string config_path(double beta, const int* shape, const std::string& method, const std::string& folder,
                         const std::string& id);

@@ -8,6 +8,9 @@
 //   --mode=qdensity    MeasureQdensityDistribution
 //   --mode=resume --pickup=T   ResumeFlowedConfiguration
 //   --mode=measured    MeasureFlowedGaugeConfigurations
+//   --checkpoint-every=N   (flow, resume) every N steps, a multiple of the
+//                      measurement interval, append the observables so far to
+//                      the CSVs and save the configuration as <id>_Flowtime<t>
 //   --precision=fp64|fp32      flow arithmetic (fp64 default)
 //   --measure=fp64|native      clover arithmetic of measurements (fp64 default)
 //   --input-method=NAME        input update-method directory (default LHMC_3_of_0_330000)
@@ -41,6 +44,8 @@ int main(int argc, char** argv) {
             else if (a == "--measure=native") lgf_host::set_measure_flags(LGF_MEASURE_DEFAULT);
             else if (a.rfind("--input-method=", 0) == 0) lgf_host::set_input_method(val("--input-method="));
             else if (a.rfind("--device=", 0) == 0) lgf_host::set_device(std::stoi(val("--device=")));
+            else if (a.rfind("--checkpoint-every=", 0) == 0)
+                lgf_host::set_checkpoint_every(std::stoi(val("--checkpoint-every=")));
             else throw lgf_host::Error(LGF_E_ARG, "unknown option " + a);
         }
         int NrDims, extdofs, ConfigStart, ConfigStop, flowSteps, measureInterval, ThermSteps, OR_per_HB, configSep;

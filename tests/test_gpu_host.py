@@ -144,6 +144,36 @@ def test_host_resume_continues_flow(tmp_path):
     assert np.array_equal(Qb, Qa)  # the saved configuration is exact in FP64, so the continuation is too
 
 
+def test_host_checkpoints(tmp_path):
+    """--checkpoint-every: the run's files are byte for byte those of an
+    uninterrupted run (the CSVs are appended in pieces), and the checkpoint
+    configuration is the one a run stopped there saves, from which the
+    resume path continues (test_host_resume_continues_flow)"""
+    d, U = small_input()
+    shape = tuple(d["shape"])
+    a, b, c = (str(tmp_path / k) + "/" for k in "abc")
+    for f in (a, b, c):
+        place_inputs(f, shape, d["beta"], [U])
+    run_lgf(a, shape, d["beta"], 0, 20, 0.02, 5)
+    run_lgf(b, shape, d["beta"], 0, 20, 0.02, 5, "--checkpoint-every=10")
+    run_lgf(c, shape, d["beta"], 0, 10, 0.02, 5)
+    sd = "X".join(map(str, shape))
+    for obs in ("Topological_Charge", "EnergyDensity"):
+        for name in ("torus_extdof4_0.csv", "torus_extdof4_0.full.csv"):
+            pa, pb = (os.path.join(f, "Observables", obs, beta_dir(d["beta"]), sd, "GF", name) for f in (a, b))
+            assert open(pa, "rb").read() == open(pb, "rb").read(), (obs, name)
+    gf = lambda f, t: os.path.join(f, "ensembles", "su3", beta_dir(d["beta"]), sd, "GF",
+                                   "torus_extdof4_0_Flowtime%f.bin" % t)
+    assert open(gf(a, 0.4), "rb").read() == open(gf(b, 0.4), "rb").read()
+    assert open(gf(b, 0.2), "rb").read() == open(gf(c, 0.2), "rb").read()
+    assert not os.path.exists(gf(a, 0.2))
+    # checkpoints must fall on measurement steps
+    args = [LGF, "4", "4", repr(d["beta"]), *map(str, shape), "0", "0", "0", "0", "0", "0", "20", "0.02", "5", a,
+            "--checkpoint-every=7"]
+    out = subprocess.run(args, capture_output=True, text=True, timeout=120)
+    assert out.returncode != 0 and "multiple of the measurement interval" in out.stderr
+
+
 def test_host_missing_input_errors(tmp_path):
     out = subprocess.run([LGF, "4", "4", "6.0", "4", "4", "4", "4", "0", "0", "0", "0", "0", "0", "5", "0.02", "5",
                           str(tmp_path) + "/"], capture_output=True, text=True, timeout=120)
