@@ -23,7 +23,12 @@ import argparse
 import ctypes
 import json
 import os
+import re
+import shutil
+import signal
+import subprocess
 import sys
+import tempfile
 import threading
 import time
 
@@ -287,6 +292,52 @@ def cpu_baseline(budget_s=20.0):
                       "threads), no measurements, %.1f s" % (n, threads, dt)}
 
 
+def cpu_baseline_reference(U, shape=SHAPE, ranks=16, steps=(1, 5), timeout_s=300):
+    """The reference program itself (built from its own sources by
+    oracle/ref/Makefile into oracle/_ref/lgf_ref) on the same input, as
+    `ranks` MPI processes (cuts 1,1,1,1: a 2x2x2x2 decomposition).  Its
+    FlowSavedGaugeConfigurations measures at t = 0 whatever the interval, so
+    the difference between a 5-step and a 1-step run (interval larger than
+    the run) is four RK3 steps.  None when the program or MPI is missing."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "lgf_ref")
+    mpiexec = "/opt/conda/bin/mpiexec"
+    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+        return None
+    folder = tempfile.mkdtemp(prefix="lgf_refbase_") + "/"
+    try:
+        d = os.path.join(folder, "ensembles", "su3", "beta" + ("%f" % BETA).replace(".", "_"),
+                         "X".join(map(str, shape)), "LHMC_3_of_0_330000")
+        os.makedirs(d)
+        np.ascontiguousarray(U, dtype=np.float64).tofile(os.path.join(d, "torus_extdof4_0.bin"))
+        walls = []
+        for n in steps:
+            args = [mpiexec, "-n", str(ranks), exe, "flow", "4", "4", repr(BETA), *map(str, shape),
+                    "1", "1", "1", "1", "0", "0", str(n), repr(EPS), "100000", folder]
+            proc = subprocess.Popen(args, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
+                                    start_new_session=True)
+            try:
+                _, err = proc.communicate(timeout=timeout_s)
+            except subprocess.TimeoutExpired:
+                os.killpg(proc.pid, signal.SIGKILL)  # mpiexec and its ranks
+                proc.communicate()
+                return None
+            m = re.search(r"lgf_ref wall_s (\S+) ranks (\d+)", err or "")
+            if proc.returncode != 0 or not m:
+                return None
+            walls.append(float(m.group(1)))
+        per_step = (walls[1] - walls[0]) / (steps[1] - steps[0])
+        if per_step <= 0:
+            return None
+        return {"value": 4.0 * np.prod(shape) / per_step, "unit": "link-updates/s", "cores": ranks,
+                "kind": "reference",
+                "sample": "the reference's FlowSavedGaugeConfigurations on the same %s input, %d MPI ranks "
+                          "(cuts 1,1,1,1), difference of a %d-step and a %d-step run (%.1f s and %.1f s, t = 0 "
+                          "measurement in both)" % ("x".join(map(str, shape)), ranks, steps[1], steps[0], walls[1],
+                                                    walls[0])}
+    finally:
+        shutil.rmtree(folder, ignore_errors=True)
+
+
 def profile_run(which, steps):
     """Fixed workload for rocprofv3 counter runs: `steps` RK3 steps of a
     seeded random 24^4 lattice in one precision (direct launches), one clover
@@ -347,6 +398,7 @@ def main():
     src = thermalised(lgf, ctx, SHAPE, BETA, SEED, THERM_SWEEPS)
     ctx.synchronize()
     plaq = src.diagnostics()[0]
+    U_in = src.download() if not args.no_cpu else None  # the reference CPU baseline's input
     log("input: 24^4 beta 6.0, %d HB/OR iterations in %.1f s, <P> = %.5f" % (THERM_SWEEPS, time.perf_counter() - tg,
                                                                              plaq))
     res = {}
@@ -413,7 +465,12 @@ def main():
         line["serving_policy_all"] = serving(lgf, args.serve_requests, value, policy=lgf.SERVE_ALL)
         log("serving (all jobs step): %s" % json.dumps(line["serving_policy_all"]))
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline()
+        port = cpu_baseline()
+        ref = cpu_baseline_reference(U_in)
+        line["cpu_baseline"] = ref if ref else port
+        if ref:
+            line["cpu_baseline_port"] = port
+        log("cpu baseline: %s" % json.dumps(line["cpu_baseline"]))
     print(json.dumps(line), flush=True)
 
 
