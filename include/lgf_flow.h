@@ -102,8 +102,9 @@ int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
 int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
 int lgf_num_stage_variants(int precision, int* n);
 /* traversal order of the lattice tiles in batches created afterwards:
- * 0 = lexicographic, B > 1 = x0 blocked by B (better cache reuse of the
- * x0-neighbours); results do not depend on it */
+ * 0 = lexicographic, B in 2..64 = x0 blocked by B (better cache reuse of
+ * the x0-neighbours), 100*B2 + B0 = x0 blocked by B0 and x2 by B2 (for
+ * lattices whose x2-x3 planes are large); results do not depend on it */
 int lgf_ctx_set_tile_order(lgf_ctx* ctx, int block);
 int lgf_ctx_get_tile_order(lgf_ctx* ctx, int* block);
 /* device scratch a measurement may hold for its line products (2- and
@@ -112,7 +113,7 @@ int lgf_ctx_get_tile_order(lgf_ctx* ctx, int* block);
  * that fit `bytes`, a single larger field on its own.  Default 1 GiB.  The
  * scratch is kept by the context until it is destroyed. */
 int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes);
-/* choose the tile order (fastest of 0, 2, 4, 8 with variant 0), then time
+/* choose the tile order (fastest of 0, 2, 4, 8, 404, 408 with variant 0), then time
  * every stage variant with it for nsteps RK3 steps on a seeded lattice of
  * `shape` (each candidate three times, its fastest run counting), and select
  * the fastest for this context; ms_per_step (optional) gets one entry per

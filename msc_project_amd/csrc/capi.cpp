@@ -442,11 +442,24 @@ extern "C" int lgf_diagnostics(lgf_field* f, double* plaq, double* umax, double*
 // away for all but one x0 in B, and the x1-neighbour B*N2*N3 sites away.
 // B = 0 keeps lexicographic order.  Reductions are indexed by the tile,
 // not by its place in this order, so results do not depend on B.
-static std::vector<int> batch_tile_order(const lgf_field* f, int B) {
+// Tile order code: 0 = lexicographic; B in 2..64 = x0 blocked by B; 100 B2 + B0
+// (B0 in 1..64, B2 in 2..64) = x0 blocked by B0 and x2 by B2.  A block of
+// x0 (and x2) values is swept along x1, so the neighbour rows a tile reads
+// along x0, x1 (and x2) were read a few tiles earlier: with both blocked the
+// sweep's window stays within an XCD's L2 also when one x2-x3 plane does not.
+static bool tile_order_valid(int code) {
+    if (code >= 0 && code <= 64) return true;
+    const int b0 = code % 100, b2 = code / 100;
+    return code > 100 && b0 >= 1 && b0 <= 64 && b2 >= 2 && b2 <= 64;
+}
+
+static std::vector<int> batch_tile_order(const lgf_field* f, int code) {
     std::vector<int> order(f->ntiles);
     for (int t = 0; t < f->ntiles; ++t) order[t] = t;
-    if (B <= 1) return order;
+    if (code <= 1) return order;
+    const int64_t B0 = code > 100 ? code % 100 : code, B2 = code > 100 ? code / 100 : 1;
     const int64_t n1 = f->shape[1], n2 = f->shape[2], n3 = f->shape[3];
+    const int64_t nb2 = (n2 + B2 - 1) / B2;
     auto key = [&](int t) {
         int64_t s = (int64_t)t * kTileSites;
         const int64_t x3 = s % n3;
@@ -455,14 +468,14 @@ static std::vector<int> batch_tile_order(const lgf_field* f, int B) {
         s /= n2;
         const int64_t x1 = s % n1;
         const int64_t x0 = s / n1;
-        return ((((x0 / B) * n1 + x1) * B + x0 % B) * n2 + x2) * n3 + x3;
+        return (((((x0 / B0) * nb2 + x2 / B2) * n1 + x1) * B0 + x0 % B0) * B2 + x2 % B2) * n3 + x3;
     };
     std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key(a) < key(b); });
     return order;
 }
 
 extern "C" int lgf_ctx_set_tile_order(lgf_ctx* ctx, int block) {
-    if (!ctx || block < 0 || block > 64) return LGF_E_ARG;
+    if (!ctx || !tile_order_valid(block)) return LGF_E_ARG;
     ctx->tile_block = block;
     return LGF_OK;
 }
@@ -826,11 +839,12 @@ extern "C" int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int
     const double eps = 0.01;
     // 1. traversal order, with the default variant
     {
-        static const int orders[4] = {0, 2, 4, 8};
+        static const int orders[] = {0, 2, 4, 8, 404, 408};
+        constexpr int norders = sizeof(orders) / sizeof(orders[0]);
         int best_order = ctx->tile_block;
         double best_order_ms = 1e300;
         ctx->stage_variant[precision] = 0;
-        for (int oi = 0; oi < 4 && !rc; ++oi) {
+        for (int oi = 0; oi < norders && !rc; ++oi) {
             ctx->tile_block = orders[oi];
             lgf_batch* bo = nullptr;
             rc = lgf_batch_create(ctx, &f, 1, &bo);

@@ -389,11 +389,11 @@ def test_stage_variants_agree_and_autotune(ctx, shape):
         best, ms = ctx.autotune(shape, prec, 2)
         assert 0 <= best < len(ms) and all(m > 0 for m in ms)
         assert ctx.stage_variant(prec) == best
-        assert ctx.tile_order() in (0, 2, 4, 8)
+        assert ctx.tile_order() in (0, 2, 4, 8, 404, 408)
         ctx.set_stage_variant(prec, 0)
     # the traversal order changes neither links nor observables
     res = []
-    for order in (0, 2, 4):
+    for order in (0, 2, 4, 404, 203):
         ctx.set_tile_order(order)
         f = lgf.Field(ctx, shape, lgf.FP64)
         f.upload(U)
