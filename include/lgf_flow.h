@@ -97,7 +97,7 @@ int lgf_ctx_synchronize(lgf_ctx* ctx);
  * LGF_FP32_SHARED_PLAQUETTE_VARIANTS compute each plaquette once for the
  * four links of a site; the others sum each link's staples in the FP64
  * path's order.  The two groups agree to FP32 rounding, not bit for bit. */
-#define LGF_FP32_SHARED_PLAQUETTE_VARIANTS 5
+#define LGF_FP32_SHARED_PLAQUETTE_VARIANTS 6
 int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
 int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
 int lgf_num_stage_variants(int precision, int* n);

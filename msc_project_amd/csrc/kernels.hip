@@ -276,7 +276,7 @@ LGF_DEV void add_planes(const PlaqLds<C>& sp, int mu, int lane, C* z) {
     }
 }
 
-template <typename S, typename C, int STAGE, int WPE, bool ORDERED, bool LOC>
+template <typename S, typename C, int STAGE, int WPE, bool ORDERED, bool LOC, bool NTS = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage_pl(
     const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int ping) {
     static_assert(std::is_same<S, float>::value && std::is_same<C, float>::value, "mixed-precision path only");
@@ -317,7 +317,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 
     M3<C> E;
     if (STAGE == 0) {
-        if (live) store_ta<S, C>(Xa, st, mu, s, Z);
+        if (live) store_ta<S, C, NTS>(Xa, st, mu, s, Z);
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
     } else if (STAGE == 1) {
@@ -325,7 +325,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
         M3<C> X1;
 #pragma unroll
         for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
-        if (live) store_ta<S, C>(Xa, st, mu, s, X1);
+        if (live) store_ta<S, C, NTS>(Xa, st, mu, s, X1);
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
     } else {
@@ -341,9 +341,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 #pragma unroll
         for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
         reunitarize<true>(Ud);
-        if (live) store_link<S, double>(Un, st, mu, s, Ud);
+        if (live) store_link<S, double, NTS>(Un, st, mu, s, Ud);
     } else {
-        if (live) store_link<S, C>(Un, st, mu, s, Unew);
+        if (live) store_link<S, C, NTS>(Un, st, mu, s, Unew);
     }
 }
 
@@ -885,27 +885,28 @@ static hipError_t stage_launch(int stage, const FieldDesc* fds, const int2* tile
     return hipGetLastError();
 }
 
-template <int WPE, bool ORDERED, bool LOC = false>
+template <int WPE, bool ORDERED, bool LOC = false, bool NTS = false>
 static hipError_t stage_launch_pl(int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
                                   hipStream_t stream) {
     dim3 grid(ntiles), block(kTileSites, 4);
     if (stage == 0)
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 0, WPE, ORDERED, LOC>), grid, block, 0, stream, fds, tiles,
+        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 0, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles,
                            ping);
     else if (stage == 1)
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 1, WPE, ORDERED, LOC>), grid, block, 0, stream, fds, tiles,
+        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 1, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles,
                            ping);
     else
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 2, WPE, ORDERED, LOC>), grid, block, 0, stream, fds, tiles,
+        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 2, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles,
                            ping);
     return hipGetLastError();
 }
 
 // Stage-kernel variants: (register budget as minimum waves per SIMD, launch
-// shape).  Mixed-precision variants 0..4 are the shared-plaquette kernel
-// (0 and 1 with the site's links in LDS), 5..10 the per-link kernel.  Variant 0 is the default (a budget the kernel
+// shape).  Mixed-precision variants 0..5 are the shared-plaquette kernel
+// (0, 1 and 5 with the site's links in LDS, 5 with non-temporal stores),
+// 6..11 the per-link kernel.  Variant 0 is the default (a budget the kernel
 // fits without spilling); lgf_autotune picks the fastest.
-int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 11; }
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 12; }
 
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream) {
@@ -923,12 +924,13 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
         case 2: return stage_launch_pl<4, true>(stage, fds, tiles, ntiles, ping, stream);
         case 3: return stage_launch_pl<4, false>(stage, fds, tiles, ntiles, ping, stream);
         case 4: return stage_launch_pl<5, true>(stage, fds, tiles, ntiles, ping, stream);
-        case 5: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 6: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 7: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 8: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 9: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
-        case 10: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
+        case 5: return stage_launch_pl<4, true, true, true>(stage, fds, tiles, ntiles, ping, stream);
+        case 6: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 7: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 8: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 9: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 10: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
+        case 11: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
         default: return stage_launch_pl<4, true, true>(stage, fds, tiles, ntiles, ping, stream);
     }
 }

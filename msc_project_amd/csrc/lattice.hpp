@@ -119,6 +119,13 @@ template <typename V, typename S>
 LGF_DEV void st_(S* plane, uint32_t byte_off, const V& v) {
     *(gptr<V>)((__attribute__((address_space(1))) char*)plane + byte_off) = v;
 }
+// the same store marked non-temporal: data written once per sweep that no
+// other wave of the sweep reads (streaming, so it does not displace reused
+// neighbour links from L2)
+template <typename V, typename S>
+LGF_DEV void st_nt(S* plane, uint32_t byte_off, const V& v) {
+    __builtin_nontemporal_store(v, (gptr<V>)((__attribute__((address_space(1))) char*)plane + byte_off));
+}
 
 // element offset of component c of link (mu, s): the layout described above
 __host__ __device__ inline int64_t link_elem_offset(bool fp32, int64_t stride, int mu, int64_t s, int c) {
@@ -156,10 +163,17 @@ LGF_DEV M3<C> load_link(const S* __restrict__ base, int64_t stride, int mu, int 
     return m;
 }
 
-template <typename S, typename C>
+template <typename S, typename C, bool NT = false>
 LGF_DEV void store_link(S* __restrict__ base, int64_t stride, int mu, int s, const M3<C>& m) {
     S* p = base + (int64_t)mu * 18 * stride;
-    if constexpr (sizeof(S) == 4) {
+    if constexpr (sizeof(S) == 4 && NT) {
+        const uint32_t o4 = (uint32_t)s * 16u, o2 = (uint32_t)s * 8u;
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            st_nt(p + (int64_t)k * 4 * stride, o4,
+                  vf4{(float)m.v[4 * k], (float)m.v[4 * k + 1], (float)m.v[4 * k + 2], (float)m.v[4 * k + 3]});
+        st_nt(p + 16 * stride, o2, vf2{(float)m.v[16], (float)m.v[17]});
+    } else if constexpr (sizeof(S) == 4) {
         const uint32_t o4 = (uint32_t)s * 16u, o2 = (uint32_t)s * 8u;
 #pragma unroll
         for (int k = 0; k < 4; ++k)
@@ -218,10 +232,14 @@ LGF_DEV M3<C> load_ta(const S* __restrict__ base, int64_t stride, int mu, int s)
     return expand_ta(a);
 }
 
-template <typename S, typename C>
+template <typename S, typename C, bool NT = false>
 LGF_DEV void store_ta(S* __restrict__ base, int64_t stride, int mu, int s, const M3<C>& m) {
     S* p = base + (int64_t)mu * 9 * stride;
-    if constexpr (sizeof(S) == 4) {
+    if constexpr (sizeof(S) == 4 && NT) {
+        st_nt(p, (uint32_t)s * 16u, vf4{(float)m.v[1], (float)m.v[9], (float)m.v[17], (float)m.v[2]});
+        st_nt(p + 4 * stride, (uint32_t)s * 16u, vf4{(float)m.v[3], (float)m.v[4], (float)m.v[5], (float)m.v[10]});
+        st_nt(p + 8 * stride, (uint32_t)s * 4u, (float)m.v[11]);
+    } else if constexpr (sizeof(S) == 4) {
         st_(p, (uint32_t)s * 16u, vf4{(float)m.v[1], (float)m.v[9], (float)m.v[17], (float)m.v[2]});
         st_(p + 4 * stride, (uint32_t)s * 16u, vf4{(float)m.v[3], (float)m.v[4], (float)m.v[5], (float)m.v[10]});
         st_(p + 8 * stride, (uint32_t)s * 4u, (float)m.v[11]);

@@ -23,7 +23,7 @@ MEASURE_FP64 = 1
 # mixed-precision stage variants below this one share each site's plaquettes
 # between its four links; the others sum each link's staples in the FP64
 # path's order (LGF_FP32_SHARED_PLAQUETTE_VARIANTS)
-FP32_SHARED_PLAQUETTE_VARIANTS = 5
+FP32_SHARED_PLAQUETTE_VARIANTS = 6
 SERVE_ALL = 0             # every admitted job steps in every round
 SERVE_SHORTEST_FIRST = 1  # least remaining work first, at least min_sites per round
 # lgf_observable order (include/lgf_flow.h)
