@@ -96,8 +96,13 @@ int lgf_ctx_synchronize(lgf_ctx* ctx);
  * context for one precision; 0 = default.  Mixed-precision variants below
  * LGF_FP32_SHARED_PLAQUETTE_VARIANTS compute each plaquette once for the
  * four links of a site; the others sum each link's staples in the FP64
- * path's order.  The two groups agree to FP32 rounding, not bit for bit. */
-#define LGF_FP32_SHARED_PLAQUETTE_VARIANTS 6
+ * path's order.  Of the shared-plaquette variants, those from
+ * LGF_FP32_COMPACT_VARIANTS on also share the backward links and rebuild
+ * row 2 of the shared links from rows 0 and 1.  Variants of one of these
+ * three groups compute bitwise identical links; the groups agree to FP32
+ * rounding. */
+#define LGF_FP32_SHARED_PLAQUETTE_VARIANTS 8
+#define LGF_FP32_COMPACT_VARIANTS 6
 int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
 int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
 int lgf_num_stage_variants(int precision, int* n);

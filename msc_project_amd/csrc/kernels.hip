@@ -158,6 +158,45 @@ struct LocLds {
     vf2 r[4][kTileSites];
 };
 
+// Everything of link (x,mu) after its flow gradient is known (nine-real
+// form z): accumulator update, exponential, left multiplication, store.
+template <typename S, typename C, int STAGE, bool NTS>
+LGF_DEV void stage_finish(const C* z, const M3<C>& Umu, S* __restrict__ Un, S* __restrict__ Xa, int64_t st, int mu,
+                          int s, bool live, C eps) {
+    const M3<C> Z = expand_ta(z);
+
+    M3<C> E;
+    if (STAGE == 0) {
+        if (live) store_ta<S, C, NTS>(Xa, st, mu, s, Z);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
+    } else if (STAGE == 1) {
+        const M3<C> X0 = load_ta<S, C>(Xa, st, mu, s);
+        M3<C> X1;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
+        if (live) store_ta<S, C, NTS>(Xa, st, mu, s, X1);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
+    } else {
+        const M3<C> X1 = load_ta<S, C>(Xa, st, mu, s);
+#pragma unroll
+        for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
+    }
+    const M3<C> ex = exp_ah_series<C>(E);
+    const M3<C> Unew = su3_nn(ex, Umu);
+    if constexpr (STAGE == 2) {
+        // once per RK3 step: project back onto SU(3) in double precision
+        M3<double> Ud;
+#pragma unroll
+        for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
+        reunitarize<true>(Ud);
+        if (live) store_link<S, double, NTS>(Un, st, mu, s, Ud);
+    } else {
+        if (live) store_link<S, C, NTS>(Un, st, mu, s, Unew);
+    }
+}
+
 template <bool ON>
 struct LocSlot {
     LocLds b;
@@ -313,38 +352,155 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     }
     __syncthreads();
     add_planes(sp, mu, lane, z);
-    const M3<C> Z = expand_ta(z);
+    stage_finish<S, C, STAGE, NTS>(z, Umu, Un, Xa, st, mu, s, live, eps);
+}
 
-    M3<C> E;
-    if (STAGE == 0) {
-        if (live) store_ta<S, C, NTS>(Xa, st, mu, s, Z);
+// ---------------------------------------------------------------------------
+// RK3 stage with shared plaquettes and shared backward links (mixed precision)
+// ---------------------------------------------------------------------------
+// For lattices that stream from HBM the shared-plaquette stage waits on L2
+// misses, so this form also shares the backward links U_nu(x - nu), which the
+// lower terms of three links of a site read, and keeps the workgroup's LDS
+// within a quarter of a CU's (four workgroups resident):
+//   - the plaquette projections as eight reals: the third imaginary diagonal
+//     entry is minus the sum of the other two (traceless);
+//   - rows 0 and 1 of the site's own links and of its backward links; row 2
+//     is rebuilt as the conjugated cross product where a whole matrix is
+//     needed (exact for SU(3), a rounding-level difference for the FP32
+//     links between projections).
+// 236 -> 180 link loads per site, about 110 more VALU instructions per wave.
+struct CompactLds {
+    float p[6][8][kTileSites];     // -1/2 TA(P_ab(x)): components 0, 1, 3..8 of the nine-real form
+    vf4 loc[4][3][kTileSites];     // rows 0, 1 of U_d(x)
+    vf4 back[4][3][kTileSites];    // rows 0, 1 of U_d(x - d)
+};
+
+LGF_DEV void rows01_store(vf4 (*q)[kTileSites], int lane, const M3<float>& m) {
 #pragma unroll
-        for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
-    } else if (STAGE == 1) {
-        const M3<C> X0 = load_ta<S, C>(Xa, st, mu, s);
-        M3<C> X1;
+    for (int k = 0; k < 3; ++k) q[k][lane] = vf4{m.v[4 * k], m.v[4 * k + 1], m.v[4 * k + 2], m.v[4 * k + 3]};
+}
+
+// rows 0, 1 from LDS; row 2 left zero (a left factor of su3_nn / su3_nd)
+// or rebuilt (WHOLE)
+template <bool WHOLE>
+LGF_DEV M3<float> rows01_load(const vf4 (*q)[kTileSites], int lane) {
+    M3<float> m;
 #pragma unroll
-        for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
-        if (live) store_ta<S, C, NTS>(Xa, st, mu, s, X1);
-#pragma unroll
-        for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
-    } else {
-        const M3<C> X1 = load_ta<S, C>(Xa, st, mu, s);
-#pragma unroll
-        for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
+    for (int k = 0; k < 3; ++k) {
+        const vf4 v = q[k][lane];
+        m.v[4 * k] = v.x, m.v[4 * k + 1] = v.y, m.v[4 * k + 2] = v.z, m.v[4 * k + 3] = v.w;
     }
-    const M3<C> ex = exp_ah_series<C>(E);
-    const M3<C> Unew = su3_nn(ex, Umu);
-    if constexpr (STAGE == 2) {
-        // once per RK3 step: project back onto SU(3) in double precision
-        M3<double> Ud;
-#pragma unroll
-        for (int i = 0; i < 18; ++i) Ud.v[i] = (double)Unew.v[i];
-        reunitarize<true>(Ud);
-        if (live) store_link<S, double, NTS>(Un, st, mu, s, Ud);
+    if constexpr (WHOLE) {
+        su3_third_row(m);
     } else {
-        if (live) store_link<S, C, NTS>(Un, st, mu, s, Unew);
+#pragma unroll
+        for (int i = 12; i < 18; ++i) m.v[i] = 0.f;
     }
+    return m;
+}
+
+template <typename S, int A, int B>
+LGF_DEV float plaquette_compact(CompactLds& L, const S* __restrict__ U, int64_t st, const Geo& g, int lane) {
+    const int s = g.s;
+    const int fa = nb_off(g, A, +1), fb = nb_off(g, B, +1);
+    const M3<float> h1 = su3_nn(rows01_load<false>(L.loc[A], lane), load_link<S, float>(U, st, B, s + fa));
+    const M3<float> h2 = su3_nn(rows01_load<false>(L.loc[B], lane), load_link<S, float>(U, st, A, s + fb));
+    float t[9];
+    ta_pack(ta_of_product_nd(h1, h2, -0.5f), t);
+    constexpr int p = plane_index(A, B);
+    L.p[p][0][lane] = t[0];
+    L.p[p][1][lane] = t[1];
+#pragma unroll
+    for (int k = 3; k < 9; ++k) L.p[p][k - 1][lane] = t[k];
+    return t[8];
+}
+
+// lower term of the (MU,NU) plane with the backward link from LDS
+template <typename S, int MU, int NU>
+LGF_DEV void lower_term_compact(M3<float>& A, const CompactLds& L, const S* __restrict__ U, int64_t st, const Geo& g,
+                                int fmu, int lane) {
+    const int s = g.s;
+    const int bnu = nb_off(g, NU, -1);
+    const M3<float> tt = su3_nn(load_link<S, float>(U, st, MU, s + bnu), load_link<S, float>(U, st, NU, s + bnu + fmu));
+    mul_dn_acc(A, tt, rows01_load<true>(L.back[NU], lane));
+}
+
+template <typename S, int MU>
+LGF_DEV void stage_pre_pb(CompactLds& L, const S* __restrict__ U, int64_t st, Geo g, int lane, float* z, M3<float>& Umu) {
+    float dep;
+    if constexpr (MU == 0) {
+        dep = plaquette_compact<S, 0, 1>(L, U, st, g, lane);
+        after<true>(g, dep);
+        dep = plaquette_compact<S, 0, 2>(L, U, st, g, lane);
+    } else if constexpr (MU == 1) {
+        dep = plaquette_compact<S, 1, 2>(L, U, st, g, lane);
+        after<true>(g, dep);
+        dep = plaquette_compact<S, 1, 3>(L, U, st, g, lane);
+    } else if constexpr (MU == 2) {
+        dep = plaquette_compact<S, 2, 3>(L, U, st, g, lane);
+    } else {
+        dep = plaquette_compact<S, 0, 3>(L, U, st, g, lane);
+    }
+    after<true>(g, dep);
+    constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
+    const int fmu = nb_off(g, MU, +1);
+    M3<float> A;
+    set_zero(A);
+    lower_term_compact<S, MU, N0>(A, L, U, st, g, fmu, lane);
+    after<true>(g, A.v[17]);
+    lower_term_compact<S, MU, N1>(A, L, U, st, g, fmu, lane);
+    after<true>(g, A.v[17]);
+    lower_term_compact<S, MU, N2>(A, L, U, st, g, fmu, lane);
+    Umu = rows01_load<true>(L.loc[MU], lane);
+    ta_pack(ta_of_product(Umu, A, -0.5f), z);
+}
+
+template <typename S, typename C, int STAGE, int WPE, bool NTS>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage_pb(
+    const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int ping) {
+    static_assert(std::is_same<S, float>::value && std::is_same<C, float>::value, "mixed-precision path only");
+    __shared__ CompactLds L;
+    const int t = xcd_remap(blockIdx.x, gridDim.x);
+    const int2 tm = tiles[t];
+    const FieldDesc& d = fds[tm.x];
+    const int lane = threadIdx.x;
+    const int s0 = tm.y * kTileSites + lane;
+    const bool live = s0 < d.vol;
+    const int s = live ? s0 : d.vol - 1;  // lanes past the end of a ragged tile compute a copy, store nothing
+    const int mu = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int cur = ping ^ d.flip;
+    const S* __restrict__ U = (const S*)d.links[cur];
+    S* __restrict__ Un = (S*)d.links[cur ^ 1];
+    S* __restrict__ Xa = (S*)d.acc;
+    const int64_t st = d.stride;
+    const Geo g = make_geo(d, s);
+    const C eps = (C)d.eps;
+
+    // each direction wave stages rows 0, 1 of its link and of its backward link
+    rows01_store(L.loc[mu], lane, load_link<S, C>(U, st, mu, s));
+    rows01_store(L.back[mu], lane, load_link<S, C>(U, st, mu, s + nb_off(g, mu, -1)));
+    __syncthreads();
+    C z[9];
+    M3<C> Umu;
+    switch (mu) {
+        case 0: stage_pre_pb<S, 0>(L, U, st, g, lane, z, Umu); break;
+        case 1: stage_pre_pb<S, 1>(L, U, st, g, lane, z, Umu); break;
+        case 2: stage_pre_pb<S, 2>(L, U, st, g, lane, z, Umu); break;
+        default: stage_pre_pb<S, 3>(L, U, st, g, lane, z, Umu); break;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int nu = k + (k >= mu ? 1 : 0);
+        const int p = mu < nu ? plane_index(mu, nu) : plane_index(nu, mu);
+        const C sg = mu < nu ? C(1) : C(-1);
+        z[0] += sg * L.p[p][0][lane];
+        z[1] += sg * L.p[p][1][lane];
+#pragma unroll
+        for (int c = 3; c < 9; ++c) z[c] += sg * L.p[p][c - 1][lane];
+    }
+    z[2] = -(z[0] + z[1]);  // the sum is traceless
+    stage_finish<S, C, STAGE, NTS>(z, Umu, Un, Xa, st, mu, s, live, eps);
 }
 
 // ---------------------------------------------------------------------------
@@ -885,6 +1041,19 @@ static hipError_t stage_launch(int stage, const FieldDesc* fds, const int2* tile
     return hipGetLastError();
 }
 
+template <bool NTS>
+static hipError_t stage_launch_pb(int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
+                                  hipStream_t stream) {
+    dim3 grid(ntiles), block(kTileSites, 4);
+    if (stage == 0)
+        hipLaunchKernelGGL((k_rk3_stage_pb<float, float, 0, 4, NTS>), grid, block, 0, stream, fds, tiles, ping);
+    else if (stage == 1)
+        hipLaunchKernelGGL((k_rk3_stage_pb<float, float, 1, 4, NTS>), grid, block, 0, stream, fds, tiles, ping);
+    else
+        hipLaunchKernelGGL((k_rk3_stage_pb<float, float, 2, 4, NTS>), grid, block, 0, stream, fds, tiles, ping);
+    return hipGetLastError();
+}
+
 template <int WPE, bool ORDERED, bool LOC = false, bool NTS = false>
 static hipError_t stage_launch_pl(int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
                                   hipStream_t stream) {
@@ -904,9 +1073,10 @@ static hipError_t stage_launch_pl(int stage, const FieldDesc* fds, const int2* t
 // Stage-kernel variants: (register budget as minimum waves per SIMD, launch
 // shape).  Mixed-precision variants 0..5 are the shared-plaquette kernel
 // (0, 1 and 5 with the site's links in LDS, 5 with non-temporal stores),
-// 6..11 the per-link kernel.  Variant 0 is the default (a budget the kernel
+// 6 and 7 the shared-plaquette kernel that also shares the backward links
+// (7 with non-temporal stores), 8..13 the per-link kernel.  Variant 0 is the default (a budget the kernel
 // fits without spilling); lgf_autotune picks the fastest.
-int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 12; }
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 14; }
 
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream) {
@@ -925,12 +1095,14 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
         case 3: return stage_launch_pl<4, false>(stage, fds, tiles, ntiles, ping, stream);
         case 4: return stage_launch_pl<5, true>(stage, fds, tiles, ntiles, ping, stream);
         case 5: return stage_launch_pl<4, true, true, true>(stage, fds, tiles, ntiles, ping, stream);
-        case 6: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 7: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 8: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 9: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
-        case 10: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
-        case 11: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
+        case 6: return stage_launch_pb<false>(stage, fds, tiles, ntiles, ping, stream);
+        case 7: return stage_launch_pb<true>(stage, fds, tiles, ntiles, ping, stream);
+        case 8: return stage_launch<float, float, 1, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 9: return stage_launch<float, float, 1, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 10: return stage_launch<float, float, 1, 5, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 11: return stage_launch<float, float, 1, 6, 4>(stage, fds, tiles, ntiles, ping, stream);
+        case 12: return stage_launch<float, float, 1, 5, 1>(stage, fds, tiles, ntiles, ping, stream);
+        case 13: return stage_launch<float, float, 1, 6, 1>(stage, fds, tiles, ntiles, ping, stream);
         default: return stage_launch_pl<4, true, true>(stage, fds, tiles, ntiles, ping, stream);
     }
 }

@@ -22,8 +22,14 @@ MEASURE_DEFAULT = 0
 MEASURE_FP64 = 1
 # mixed-precision stage variants below this one share each site's plaquettes
 # between its four links; the others sum each link's staples in the FP64
-# path's order (LGF_FP32_SHARED_PLAQUETTE_VARIANTS)
-FP32_SHARED_PLAQUETTE_VARIANTS = 6
+# path's order (LGF_FP32_SHARED_PLAQUETTE_VARIANTS); from FP32_COMPACT_VARIANTS
+# on the shared-plaquette variants also share the backward links and rebuild
+# row 2 of shared links (LGF_FP32_COMPACT_VARIANTS)
+FP32_SHARED_PLAQUETTE_VARIANTS = 8
+FP32_COMPACT_VARIANTS = 6
+# variant ranges whose links agree bit for bit
+FP32_VARIANT_GROUPS = ((0, FP32_COMPACT_VARIANTS), (FP32_COMPACT_VARIANTS, FP32_SHARED_PLAQUETTE_VARIANTS),
+                       (FP32_SHARED_PLAQUETTE_VARIANTS, None))
 SERVE_ALL = 0             # every admitted job steps in every round
 SERVE_SHORTEST_FIRST = 1  # least remaining work first, at least min_sites per round
 # lgf_observable order (include/lgf_flow.h)

@@ -163,7 +163,7 @@ def test_flow_keeps_constant_abelian_field(ctx):
     shape = (6, 4, 4, 6)
     U, p, q = twisted_flux_field(shape, 1, -2)
     for prec, tol in ((lgf.FP64, 1e-13), (lgf.FP32, 2e-6)):
-        variants = [0, lgf.FP32_SHARED_PLAQUETTE_VARIANTS] if prec == lgf.FP32 else [0]
+        variants = [0, lgf.FP32_COMPACT_VARIANTS, lgf.FP32_SHARED_PLAQUETTE_VARIANTS] if prec == lgf.FP32 else [0]
         for v in variants:
             ctx.set_stage_variant(prec, v)
             f = lgf.Field(ctx, shape, prec)
@@ -323,7 +323,7 @@ def test_bad_shape_rejected(ctx):
 
 
 @pytest.mark.parametrize("shape", [(4, 4, 6, 4), (2, 4, 2, 6), (6, 2, 4, 2)])
-@pytest.mark.parametrize("variant", [0, lgf.FP32_SHARED_PLAQUETTE_VARIANTS])
+@pytest.mark.parametrize("variant", [0, lgf.FP32_COMPACT_VARIANTS, lgf.FP32_SHARED_PLAQUETTE_VARIANTS])
 def test_fp32_flow_tracks_oracle_on_hot_start(ctx, variant, shape):
     """the mixed-precision flow (7-term series exponential, FP64 projection
     per step) stays within FP32 rounding of the FP64 oracle on a rough hot
@@ -344,7 +344,7 @@ def test_fp32_flow_tracks_oracle_on_hot_start(ctx, variant, shape):
     ctx.set_stage_variant(lgf.FP32, 0)
 
 
-@pytest.mark.parametrize("variant", [0, lgf.FP32_SHARED_PLAQUETTE_VARIANTS])
+@pytest.mark.parametrize("variant", [0, lgf.FP32_COMPACT_VARIANTS, lgf.FP32_SHARED_PLAQUETTE_VARIANTS])
 def test_fp32_tracks_fp64(ctx, config1, variant):
     """mixed precision vs the FP64 path on the config-1 inputs"""
     d, shape, cfgs = config1
@@ -379,13 +379,14 @@ def test_stage_variants_agree_and_autotune(ctx, shape):
             f.flow(0.02, 3)
             outs.append(f.download())
             f.close()
-        # variants in one group compute the same links bit for bit
-        npl = lgf.FP32_SHARED_PLAQUETTE_VARIANTS if prec == lgf.FP32 else 0
-        for grp in (outs[:npl], outs[npl:]):
+        # variants in one group compute the same links bit for bit, the
+        # groups agree to FP32 rounding
+        groups = lgf.FP32_VARIANT_GROUPS if prec == lgf.FP32 else ((0, None),)
+        for a, b in groups:
+            grp = outs[a:b]
             for o in grp[1:]:
-                assert np.array_equal(o, grp[0])
-        if npl:
-            assert np.max(np.abs(outs[npl] - outs[0])) < 2e-6
+                assert np.array_equal(o, grp[0]), (prec, a, b)
+            assert np.max(np.abs(grp[0] - outs[0])) < 2e-6, (prec, a)
         best, ms = ctx.autotune(shape, prec, 2)
         assert 0 <= best < len(ms) and all(m > 0 for m in ms)
         assert ctx.stage_variant(prec) == best
