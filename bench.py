@@ -271,6 +271,15 @@ def serving(lgf, nreq, throughput, seed=4242, policy=1, age_rate=0.0):
             "batch_steps": st["batch_steps"]}
 
 
+def stage_kernel_name(lgf, variant):
+    """the mixed-precision stage kernel behind a stage variant"""
+    if variant < lgf.FP32_COMPACT_VARIANTS:
+        return "k_rk3_stage_pl<float>"
+    if variant < lgf.FP32_SHARED_PLAQUETTE_VARIANTS:
+        return "k_rk3_stage_pb<float>"
+    return "k_rk3_stage<float>"
+
+
 def cpu_baseline(budget_s=20.0):
     """The CPU oracle (a C port of the reference path, OpenMP over sites) on
     whole RK3 steps of a 24^4 lattice (seeded Haar-random links; the flow's
@@ -449,8 +458,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
                      "kernel": "%s (stage variant %d, 3 launches per RK3 step, 576 B per link-update algorithmic)" % (
-                         "k_rk3_stage_pl<float>" if r32["stage_variant"] < lgf.FP32_SHARED_PLAQUETTE_VARIANTS
-                         else "k_rk3_stage<float>", r32["stage_variant"])},
+                         stage_kernel_name(lgf, r32["stage_variant"]), r32["stage_variant"])},
     }
     if not args.no_batch:
         line["batched_16_uneven"] = batched(lgf, ctx)
