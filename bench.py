@@ -17,7 +17,7 @@ events on the library's stream), a batched run of 16 uneven lattices, a
 continuous-batching serving run with p50/p99 latency, and a CPU baseline.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 10]
-    python bench.py --profile fp32|fp64|fp32native --steps 6   (fixed workload for rocprofv3)
+    python bench.py --profile fp32|fp32pb|fp64|fp32native --steps 6   (fixed workload for rocprofv3)
 """
 import argparse
 import ctypes
@@ -355,6 +355,8 @@ def profile_run(which, steps):
     from msc_project_amd import lgf
     ctx = lgf.Context(0)
     prec = lgf.FP64 if which == "fp64" else lgf.FP32
+    if which == "fp32pb":  # the shared-backward-link stage kernel
+        ctx.set_stage_variant(lgf.FP32, lgf.FP32_COMPACT_VARIANTS)
     src = lgf.Field(ctx, SHAPE, prec)  # copy source, cold by the time it is copied
     src.set_random(99)
     f = lgf.Field(ctx, SHAPE, prec)
@@ -371,14 +373,14 @@ def profile_run(which, steps):
                       "stage_algorithmic_bytes_per_step": BYTES_PER_LU[prec] * 4 * int(np.prod(SHAPE))}))
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the FP32 stage kernel (mean of the three
+def traffic_from_profiles(kernel):
+    """HBM bytes per launch of the named FP32 stage kernel (mean of the three
     stages) from the committed rocprofv3 FETCH_SIZE/WRITE_SIZE summary."""
     p = os.path.join(ROOT, "profiles", "rk3_stage_fp32_24x24x24x24_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
             d = json.load(f)
-        return d.get("hbm_bytes_per_launch")
+        return d.get("hbm_bytes_per_launch_by_kernel", {}).get(kernel)
     return None
 
 
@@ -390,7 +392,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=40, help="0 disables the serving run")
-    ap.add_argument("--profile", choices=["fp32", "fp64", "fp32native"], default=None,
+    ap.add_argument("--profile", choices=["fp32", "fp32pb", "fp64", "fp32native"], default=None,
                     help="run the fixed counter-profiling workload instead of the benchmark")
     args = ap.parse_args()
     if args.profile:
@@ -456,7 +458,8 @@ def main():
         "tuning": {"stage_variant": {"fp64": r64["stage_variant"], "fp32": r32["stage_variant"]},
                    "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]}},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic_from_profiles(),
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(stage_kernel_name(lgf, r32["stage_variant"])),
                      "kernel": "%s (stage variant %d, 3 launches per RK3 step, 576 B per link-update algorithmic)" % (
                          stage_kernel_name(lgf, r32["stage_variant"]), r32["stage_variant"])},
     }
