@@ -195,6 +195,8 @@ def batched(lgf, ctx):
         f.copy_from(g)
         g.close()
         fields.append(f)
+    # tile order and stage variant tuned for this mix of shapes (seeded lattices, before the timed run)
+    best, _ = ctx.autotune_batch([s for s, _ in specs], lgf.FP32, nsteps=5)
     b = lgf.Batch(ctx, fields)
     b.flow(EPS, 2, use_graph=True)  # capture
     ctx.synchronize()
@@ -212,7 +214,8 @@ def batched(lgf, ctx):
     for f in fields:
         f.close()
     return {"configs": len(specs), "link_updates": lu, "wall_s": wall, "link_updates_per_s": lu / wall,
-            "steps": 300, "measure_every": 10, "precision": "fp32"}
+            "steps": 300, "measure_every": 10, "precision": "fp32",
+            "tuning": {"stage_variant": best, "tile_order": ctx.tile_order()}}
 
 
 def serving(lgf, nreq, throughput, seed=4242, policy=1, age_rate=0.0):
@@ -234,6 +237,7 @@ def serving(lgf, nreq, throughput, seed=4242, policy=1, age_rate=0.0):
     gaps = rng.exponential(1.0 / rate, size=nreq)
     srv = lgf.Server(0)
     srv.set_policy(policy, age_rate=age_rate)
+    tuned = srv.autotune(shapes, lgf.FP32, nsteps=5)  # one lattice of each shape the stream draws from
     ids = [None] * nreq
 
     def feeder():
@@ -268,7 +272,7 @@ def serving(lgf, nreq, throughput, seed=4242, policy=1, age_rate=0.0):
             "arrival_rate_per_s": rate, "all_ok": ok, "p50_latency_s": float(np.percentile(lat, 50)),
             "p99_latency_s": float(np.percentile(lat, 99)), "mean_latency_s": float(lat.mean()),
             "throughput_link_updates_per_s": lu / wall, "wall_s": wall, "busy_s": st["busy_s"],
-            "batch_steps": st["batch_steps"]}
+            "batch_steps": st["batch_steps"], "stage_variant": tuned}
 
 
 def stage_kernel_name(lgf, variant):

@@ -118,12 +118,19 @@ int lgf_ctx_get_tile_order(lgf_ctx* ctx, int* block);
  * that fit `bytes`, a single larger field on its own.  Default 1 GiB.  The
  * scratch is kept by the context until it is destroyed. */
 int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes);
-/* choose the tile order (fastest of 0, 2, 4, 8, 404, 408 with variant 0), then time
- * every stage variant with it for nsteps RK3 steps on a seeded lattice of
- * `shape` (each candidate three times, its fastest run counting), and select
- * the fastest for this context; ms_per_step (optional) gets one entry per
- * variant */
+/* time every pair of tile order (0, 2, 4, 8, 404, 408) and stage variant for
+ * nsteps RK3 steps on a seeded lattice of `shape` (each pair three times, its
+ * fastest run counting), and make the fastest pair this context's tile order
+ * and its stage variant for `precision`; ms_per_step (optional) gets one
+ * entry per variant, at the chosen order.  No counterpart in the reference
+ * (its flow has no launch parameters to choose). */
 int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best, double* ms_per_step);
+/* the same search on `nshapes` seeded lattices (shapes[4*i .. 4*i+3])
+ * flowed together as one batch: tunes the context for batches of that mix
+ * of shapes, whose best order and variant can differ from any one
+ * lattice's.  LGF_E_ARG for nshapes <= 0, LGF_E_SHAPE for a bad shape. */
+int lgf_autotune_batch(lgf_ctx* ctx, const int* shapes, int nshapes, int precision, int nsteps, int* best,
+                       double* ms_per_step);
 
 /* ---- one lattice ------------------------------------------------------ */
 /* device bytes held by a field of this shape and precision */
@@ -241,6 +248,11 @@ int lgf_server_wait(lgf_server* s, uint64_t id, double timeout_s, lgf_result* ou
  * the policy. */
 enum lgf_serve_policy { LGF_SERVE_ALL = 0, LGF_SERVE_SHORTEST_FIRST = 1 };
 int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites, double age_rate);
+/* lgf_autotune_batch on the server's own context, for the mix of lattice
+ * shapes it is expected to serve.  Only while the server is idle (every
+ * submitted job finished): LGF_E_STATE otherwise.  Jobs submitted while it
+ * runs wait for it. */
+int lgf_server_autotune(lgf_server* s, const int* shapes, int nshapes, int precision, int nsteps, int* best);
 int lgf_server_now(lgf_server* s, double* now_s);
 /* counters: jobs done, steps executed, link-updates executed, busy seconds */
 int lgf_server_stats(lgf_server* s, uint64_t* jobs_done, uint64_t* batch_steps, double* link_updates,

@@ -804,10 +804,14 @@ extern "C" int lgf_measure_finish(lgf_field* f, double* obs) {
 // ---------------------------------------------------------------------------
 // autotuner
 // ---------------------------------------------------------------------------
-// Times every stage-kernel variant on a seeded random lattice of the given
-// shape (nsteps RK3 steps each, after one warm-up step, HIP events on the
-// context's stream) and makes the fastest the context's variant for that
-// precision.  ms_per_step (optional) receives one time per variant.
+// Times every (tile order, stage-kernel variant) pair on seeded random
+// lattices of the given shapes, flowed together as one batch (nsteps RK3
+// steps per timing, after one warm-up step, HIP events on the context's
+// stream), and makes the fastest pair the context's tile order and variant
+// for that precision.  The pairs are searched jointly: the fastest order
+// depends on the variant (non-temporal stores change which neighbours stay
+// in cache).  ms_per_step (optional) receives one time per variant, at the
+// chosen order.
 // each candidate is timed kAutotuneReps times and its fastest run counts
 constexpr int kAutotuneReps = 3;
 
@@ -821,74 +825,69 @@ static int time_steps(lgf_ctx* ctx, lgf_batch* b, double eps, int nsteps, hipEve
     return LGF_OK;
 }
 
-extern "C" int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best,
-                            double* ms_per_step) {
-    if (!ctx || nsteps <= 0 || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
-    lgf_field* f = nullptr;
-    int rc = lgf_field_create(ctx, shape, precision, &f);
-    if (rc) return rc;
-    rc = lgf_field_set_random(f, 0x5eedULL);
-    lgf_batch* b = nullptr;
-    if (!rc) rc = lgf_batch_create(ctx, &f, 1, &b);
+static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int precision, int nsteps, int* best,
+                           double* ms_per_step) {
+    static const int orders[] = {0, 2, 4, 8, 404, 408};
+    constexpr int norders = sizeof(orders) / sizeof(orders[0]);
+    const int nv = num_stage_variants(precision);
+    const int keep_order = ctx->tile_block, keep_variant = ctx->stage_variant[precision];
+    std::vector<lgf_field*> fields;
+    int rc = LGF_OK;
+    for (int i = 0; i < nshapes && !rc; ++i) {
+        lgf_field* f = nullptr;
+        rc = lgf_field_create(ctx, shapes + 4 * i, precision, &f);
+        if (!rc) {
+            fields.push_back(f);
+            rc = lgf_field_set_random(f, 0x5eedULL + (uint64_t)i);
+        }
+    }
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = lgf_fail(ctx, LGF_E_HIP, "event creation");
-    const int nv = num_stage_variants(precision);
-    int arg = 0;
-    double best_ms = 1e300;
     const double eps = 0.01;
-    // 1. traversal order, with the default variant
-    {
-        static const int orders[] = {0, 2, 4, 8, 404, 408};
-        constexpr int norders = sizeof(orders) / sizeof(orders[0]);
-        int best_order = ctx->tile_block;
-        double best_order_ms = 1e300;
-        ctx->stage_variant[precision] = 0;
-        for (int oi = 0; oi < norders && !rc; ++oi) {
-            ctx->tile_block = orders[oi];
-            lgf_batch* bo = nullptr;
-            rc = lgf_batch_create(ctx, &f, 1, &bo);
-            if (!rc) rc = lgf_batch_flow_steps(bo, &eps, 1, 0);
-            float ms = 0.f;
+    std::vector<double> ms(size_t(norders) * nv, 1e300);
+    for (int oi = 0; oi < norders && !rc; ++oi) {
+        ctx->tile_block = orders[oi];
+        lgf_batch* b = nullptr;
+        rc = lgf_batch_create(ctx, fields.data(), (int)fields.size(), &b);
+        for (int v = 0; v < nv && !rc; ++v) {
+            ctx->stage_variant[precision] = v;
+            rc = lgf_batch_flow_steps(b, &eps, 1, 0);
             for (int r = 0; r < kAutotuneReps && !rc; ++r) {
                 float m = 0.f;
-                rc = time_steps(ctx, bo, eps, nsteps, e0, e1, &m);
-                if (!rc && (r == 0 || m < ms)) ms = m;
-            }
-            if (bo) lgf_batch_destroy(bo);
-            if (!rc && ms < best_order_ms) {
-                best_order_ms = ms;
-                best_order = orders[oi];
+                rc = time_steps(ctx, b, eps, nsteps, e0, e1, &m);
+                if (!rc) ms[size_t(oi) * nv + v] = std::min(ms[size_t(oi) * nv + v], (double)m / nsteps);
             }
         }
-        ctx->tile_block = best_order;
         if (b) lgf_batch_destroy(b);
-        b = nullptr;
-        if (!rc) rc = lgf_batch_create(ctx, &f, 1, &b);
     }
-    // 2. stage-kernel variant, with that order
-    for (int v = 0; v < nv && !rc; ++v) {
-        ctx->stage_variant[precision] = v;
-        rc = lgf_batch_flow_steps(b, &eps, 1, 0);
-        float ms = 0.f;
-        for (int r = 0; r < kAutotuneReps && !rc; ++r) {
-            float m = 0.f;
-            rc = time_steps(ctx, b, eps, nsteps, e0, e1, &m);
-            if (!rc && (r == 0 || m < ms)) ms = m;
-        }
-        if (rc) break;
-        const double per = (double)ms / nsteps;
-        if (ms_per_step) ms_per_step[v] = per;
-        if (per < best_ms) {
-            best_ms = per;
-            arg = v;
-        }
-    }
-    ctx->stage_variant[precision] = arg;
-    if (best) *best = arg;
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
-    if (b) lgf_batch_destroy(b);
-    lgf_field_destroy(f);
-    return rc;
+    for (lgf_field* f : fields) lgf_field_destroy(f);
+    if (rc) {
+        ctx->tile_block = keep_order;
+        ctx->stage_variant[precision] = keep_variant;
+        return rc;
+    }
+    const size_t arg = size_t(std::min_element(ms.begin(), ms.end()) - ms.begin());
+    const int bo = int(arg / nv), bv = int(arg % nv);
+    ctx->tile_block = orders[bo];
+    ctx->stage_variant[precision] = bv;
+    if (best) *best = bv;
+    if (ms_per_step)
+        for (int v = 0; v < nv; ++v) ms_per_step[v] = ms[size_t(bo) * nv + v];
+    return LGF_OK;
+}
+
+extern "C" int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best,
+                            double* ms_per_step) {
+    if (!ctx || !shape || nsteps <= 0 || (precision != LGF_FP64 && precision != LGF_FP32)) return LGF_E_ARG;
+    return autotune_shapes(ctx, shape, 1, precision, nsteps, best, ms_per_step);
+}
+
+extern "C" int lgf_autotune_batch(lgf_ctx* ctx, const int* shapes, int nshapes, int precision, int nsteps, int* best,
+                                  double* ms_per_step) {
+    if (!ctx || !shapes || nshapes <= 0 || nsteps <= 0 || (precision != LGF_FP64 && precision != LGF_FP32))
+        return LGF_E_ARG;
+    return autotune_shapes(ctx, shapes, nshapes, precision, nsteps, best, ms_per_step);
 }

@@ -81,6 +81,7 @@ struct lgf_server {
     std::map<uint64_t, JobP> finished;
     uint64_t next_id = 1;
     bool stop = false;
+    bool tuning = false;  // lgf_server_autotune is using ctx: admit nothing
     std::string err;
     std::chrono::steady_clock::time_point t0;
     // counters (guarded by mu)
@@ -192,7 +193,7 @@ void lgf_server::run() {
         std::vector<JobP> admitted;
         {
             std::unique_lock<std::mutex> lk(mu);
-            cv_work.wait(lk, [&] { return stop || !pending.empty() || !active.empty(); });
+            cv_work.wait(lk, [&] { return stop || (!pending.empty() && !tuning) || !active.empty(); });
             if (stop) break;
             if (active.empty()) gstep = 0;
             size_t used = 0, lines = 0;
@@ -200,7 +201,7 @@ void lgf_server::run() {
                 used += j->bytes;
                 lines = std::max(lines, j->line_bytes);
             }
-            while (gstep % kTick == 0 && !pending.empty()) {
+            while (gstep % kTick == 0 && !pending.empty() && !tuning) {
                 JobP j = pending.front();
                 if (used + j->bytes + measure_reserve(std::max(lines, j->line_bytes)) > budget) break;
                 if (max_active > 0 && (int)(active.size() + admitted.size()) >= max_active) break;
@@ -470,6 +471,31 @@ extern "C" int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_site
     }
     s->cv_work.notify_all();
     return LGF_OK;
+}
+
+// The server's thread touches its context only while jobs are resident, and
+// a job counts as done only after its field and batches are released; so
+// with every submitted job done, and admission held off by `tuning`, the
+// context is the caller's until the search ends.
+extern "C" int lgf_server_autotune(lgf_server* s, const int* shapes, int nshapes, int precision, int nsteps, int* best) {
+    if (!s || !shapes || nshapes <= 0 || nsteps <= 0) return LGF_E_ARG;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->stop || s->tuning || s->next_id - 1 != s->jobs_done) {
+            s->err = "autotune needs an idle server: every submitted job finished";
+            return LGF_E_STATE;
+        }
+        s->tuning = true;
+    }
+    int rc = hipSetDevice(s->device) == hipSuccess ? LGF_OK : LGF_E_HIP;
+    if (!rc) rc = lgf_autotune_batch(s->ctx, shapes, nshapes, precision, nsteps, best, nullptr);
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->tuning = false;
+        if (rc) s->err = lgf_ctx_last_error(s->ctx);
+    }
+    s->cv_work.notify_all();
+    return rc;
 }
 
 extern "C" int lgf_server_now(lgf_server* s, double* now_s) {

@@ -74,6 +74,7 @@ SIGNATURES = {
     "lgf_ctx_get_stage_variant": ([_vp, ctypes.c_int, _ip], ctypes.c_int),
     "lgf_num_stage_variants": ([ctypes.c_int, _ip], ctypes.c_int),
     "lgf_autotune": ([_vp, _ip, ctypes.c_int, ctypes.c_int, _ip, _dp], ctypes.c_int),
+    "lgf_autotune_batch": ([_vp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp], ctypes.c_int),
     "lgf_ctx_set_tile_order": ([_vp, ctypes.c_int], ctypes.c_int),
     "lgf_ctx_get_tile_order": ([_vp, _ip], ctypes.c_int),
     "lgf_ctx_set_measure_scratch": ([_vp, ctypes.c_size_t], ctypes.c_int),
@@ -110,6 +111,7 @@ SIGNATURES = {
     "lgf_server_now": ([_vp, _dp], ctypes.c_int),
     "lgf_flow_scale_t0": ([_dp, _dp, ctypes.c_int, ctypes.c_double, _dp], ctypes.c_int),
     "lgf_server_set_policy": ([_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_double], ctypes.c_int),
+    "lgf_server_autotune": ([_vp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip], ctypes.c_int),
     "lgf_server_stats": ([_vp, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(ctypes.c_uint64), _dp, _dp],
                          ctypes.c_int),
     "lgf_server_last_error": ([_vp], ctypes.c_char_p),
@@ -194,6 +196,17 @@ class Context:
         best = ctypes.c_int()
         self.check(lib().lgf_autotune(self.h, _shape(shape), precision, nsteps, ctypes.byref(best), _p(ms)),
                    "autotune")
+        return best.value, ms.tolist()
+
+    def autotune_batch(self, shapes, precision, nsteps=10):
+        """Tunes tile order and stage variant on seeded lattices of `shapes`
+        flowed as one batch; returns (best variant, ms_per_step list)."""
+        n = num_stage_variants(precision)
+        ms = np.zeros(n)
+        best = ctypes.c_int()
+        sh = np.ascontiguousarray(np.asarray(shapes, dtype=np.int32).reshape(-1, 4))
+        self.check(lib().lgf_autotune_batch(self.h, sh.ctypes.data_as(_ip), len(sh), precision, nsteps,
+                                            ctypes.byref(best), _p(ms)), "autotune_batch")
         return best.value, ms.tolist()
 
     def set_tile_order(self, block):
@@ -422,6 +435,15 @@ class Server:
     def set_policy(self, policy, min_sites=1 << 20, age_rate=0.0):
         """age_rate: link-updates of remaining work credited per second waited"""
         self._check(lib().lgf_server_set_policy(self.h, int(policy), int(min_sites), float(age_rate)), "set_policy")
+
+    def autotune(self, shapes, precision=FP32, nsteps=10):
+        """Tunes the server's context for a mix of lattice shapes (idle server
+        only); returns the chosen stage variant."""
+        best = ctypes.c_int()
+        sh = np.ascontiguousarray(np.asarray(shapes, dtype=np.int32).reshape(-1, 4))
+        self._check(lib().lgf_server_autotune(self.h, sh.ctypes.data_as(_ip), len(sh), precision, nsteps,
+                                              ctypes.byref(best)), "autotune")
+        return best.value
 
     def _check(self, rc, what):
         if rc:

@@ -406,6 +406,33 @@ def test_stage_variants_agree_and_autotune(ctx, shape):
         assert np.array_equal(r[0], res[0][0]) and r[1] == res[0][1]
 
 
+def test_autotune_batch(ctx):
+    """The batch autotuner times every (tile order, variant) pair on a mix of
+    shapes flowed as one batch and keeps the fastest; the returned times are
+    those of the chosen order, so the chosen variant is their minimum.  A bad
+    shape or an empty list is refused and leaves the context as it was."""
+    shapes = [(4, 4, 4, 8), (6, 4, 2, 2), (8, 8, 4, 4)]
+    for prec in (lgf.FP64, lgf.FP32):
+        best, ms = ctx.autotune_batch(shapes, prec, 2)
+        assert len(ms) == lgf.num_stage_variants(prec)
+        assert 0 <= best < len(ms) and all(0 < m < 1e3 for m in ms)
+        assert ms[best] == min(ms)
+        assert ctx.stage_variant(prec) == best
+        assert ctx.tile_order() in (0, 2, 4, 8, 404, 408)
+    ctx.set_stage_variant(lgf.FP32, 3)
+    ctx.set_tile_order(2)
+    with pytest.raises(lgf.LgfError) as e:
+        ctx.autotune_batch([(4, 4, 4, 4), (5, 4, 4, 4)], lgf.FP32, 2)
+    assert e.value.code == 2  # LGF_E_SHAPE
+    with pytest.raises(lgf.LgfError) as e:
+        ctx.autotune_batch(np.zeros((0, 4)), lgf.FP32, 2)
+    assert e.value.code == 1  # LGF_E_ARG
+    assert ctx.stage_variant(lgf.FP32) == 3 and ctx.tile_order() == 2
+    for prec in (lgf.FP64, lgf.FP32):
+        ctx.set_stage_variant(prec, 0)
+    ctx.set_tile_order(4)
+
+
 @pytest.mark.slow
 def test_large24_golden_fp64(ctx):
     """First 50 steps of the 24^4, beta 6.0 reference flow (eps 0.01, every 5).

@@ -76,3 +76,27 @@ def test_server_admission_by_footprint():
     assert starts[2] >= finishes[0] - 1e-6
     for r in res[1:]:
         assert np.array_equal(r["E"], res[0]["E"]) and np.array_equal(r["Q"], res[0]["Q"])
+
+
+def test_server_autotune_only_when_idle():
+    """The server's context is tuned only while no job is pending or
+    resident; a job then runs with the tuned variant, bitwise as the same
+    job flowed alone with that variant."""
+    srv = lgf.Server(0)
+    best = srv.autotune([(8, 8, 8, 8), (4, 4, 4, 8)], lgf.FP32, 2)
+    assert 0 <= best < lgf.num_stage_variants(lgf.FP32)
+    job = ((16, 16, 16, 16), 0.01, 300, 100, lgf.FP32, 21)
+    shape, eps, n, every, prec, seed = job
+    i = srv.submit(shape, eps, n, every, precision=prec, seed=seed)
+    with pytest.raises(lgf.LgfError) as e:
+        srv.autotune([(8, 8, 8, 8)], lgf.FP32, 2)
+    assert e.value.code == 7  # LGF_E_STATE: a job is in flight
+    r = srv.wait(i, timeout=120)
+    assert r["status"] == 0 and len(r["t"]) == 4
+    assert 0 <= srv.autotune([(16, 16, 16, 16)], lgf.FP32, 2)  # idle again
+    srv.close()
+    ctx = lgf.Context(0)
+    ctx.set_stage_variant(lgf.FP32, best)
+    t, E, Q = alone(ctx, shape, eps, n, every, prec, seed=seed)
+    ctx.close()
+    assert np.array_equal(r["t"], t) and np.array_equal(r["E"], E) and np.array_equal(r["Q"], Q)
