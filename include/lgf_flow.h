@@ -128,7 +128,10 @@ int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, in
 /* the same search on `nshapes` seeded lattices (shapes[4*i .. 4*i+3])
  * flowed together as one batch: tunes the context for batches of that mix
  * of shapes, whose best order and variant can differ from any one
- * lattice's.  LGF_E_ARG for nshapes <= 0, LGF_E_SHAPE for a bad shape. */
+ * lattice's.  The lattices of all shapes are held at once (the sum of their
+ * lgf_field_footprint) and released before it returns.  LGF_E_ARG for
+ * nshapes <= 0, LGF_E_SHAPE for a bad shape; on failure the context's
+ * tile order and variant are left as they were. */
 int lgf_autotune_batch(lgf_ctx* ctx, const int* shapes, int nshapes, int precision, int nsteps, int* best,
                        double* ms_per_step);
 
@@ -250,8 +253,9 @@ enum lgf_serve_policy { LGF_SERVE_ALL = 0, LGF_SERVE_SHORTEST_FIRST = 1 };
 int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites, double age_rate);
 /* lgf_autotune_batch on the server's own context, for the mix of lattice
  * shapes it is expected to serve.  Only while the server is idle (every
- * submitted job finished): LGF_E_STATE otherwise.  Jobs submitted while it
- * runs wait for it. */
+ * submitted job finished): LGF_E_STATE otherwise, also while another
+ * call of it runs.  Jobs submitted while it runs wait for it.  Not to be
+ * called concurrently with lgf_server_destroy. */
 int lgf_server_autotune(lgf_server* s, const int* shapes, int nshapes, int precision, int nsteps, int* best);
 int lgf_server_now(lgf_server* s, double* now_s);
 /* counters: jobs done, steps executed, link-updates executed, busy seconds */
