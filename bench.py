@@ -17,7 +17,8 @@ events on the library's stream), a batched run of 16 uneven lattices, a
 continuous-batching serving run with p50/p99 latency, and a CPU baseline.
 
     python bench.py [--gpus 1] [--steps 500] [--warmup 10]
-    python bench.py --profile fp32|fp32pb|fp64|fp32native --steps 6   (fixed workload for rocprofv3)
+    python bench.py --profile fp32|fp32pb|fp32stream|fp64|fp32native --steps 6   (fixed workload for rocprofv3)
+    python bench.py --profile fp32hold --steps 40000   (about 10 s of FP32 flow, for sampling the shader clock)
 """
 import argparse
 import ctypes
@@ -351,30 +352,88 @@ def cpu_baseline_reference(U, shape=SHAPE, ranks=16, steps=(1, 5), timeout_s=300
         shutil.rmtree(folder, ignore_errors=True)
 
 
+STREAM_SHAPE = (32, 32, 32, 32)  # FP32 working set 604 MB: streams from HBM
+
+
+def hold_flow(lgf, ctx, steps):
+    """Sustained FP32 load for sampling the shader clock (rocm-smi): `steps`
+    graph-replayed RK3 steps of a seeded random 24^4 lattice with the tuned
+    stage kernel, in chunks of 100, printing the step time at the end."""
+    ctx.autotune(SHAPE, lgf.FP32, 5)
+    f = lgf.Field(ctx, SHAPE, lgf.FP32)
+    f.set_random(5)
+    b = lgf.Batch(ctx, [f])
+    b.flow(0.001, 4, use_graph=True)
+    ctx.synchronize()
+    log("hold: start, variant %d, tile order %d" % (ctx.stage_variant(lgf.FP32), ctx.tile_order()))
+    t0 = time.perf_counter()
+    n = 0
+    while n < steps:
+        b.flow(0.001, 100, use_graph=True)
+        ctx.synchronize()
+        n += 100
+    dt = time.perf_counter() - t0
+    print(json.dumps({"profile": "fp32hold", "lattice": list(SHAPE), "steps": n, "seconds": dt,
+                      "ms_per_step": dt / n * 1e3, "stage_variant": ctx.stage_variant(lgf.FP32)}))
+    b.close()
+    f.close()
+
+
 def profile_run(which, steps):
     """Fixed workload for rocprofv3 counter runs: `steps` RK3 steps of a
     seeded random 24^4 lattice in one precision (direct launches), one clover
     measurement, and one plain copy of a link array (k_convert, a kernel of
-    known traffic that calibrates the counters)."""
+    known traffic that calibrates the counters).  "fp32stream" runs the same
+    on a 32^4 lattice, which does not fit the last-level cache, with the
+    stage kernel and tile order the autotuner picks for it (the
+    shared-backward-link kernel with non-temporal stores, x0/x2 blocks of 4)."""
     from msc_project_amd import lgf
     ctx = lgf.Context(0)
+    if which == "fp32hold":
+        hold_flow(lgf, ctx, steps)
+        return
     prec = lgf.FP64 if which == "fp64" else lgf.FP32
+    shape = STREAM_SHAPE if which == "fp32stream" else SHAPE
     if which == "fp32pb":  # the shared-backward-link stage kernel
         ctx.set_stage_variant(lgf.FP32, lgf.FP32_COMPACT_VARIANTS)
-    src = lgf.Field(ctx, SHAPE, prec)  # copy source, cold by the time it is copied
+    if which == "fp32stream":
+        ctx.set_stage_variant(lgf.FP32, lgf.FP32_COMPACT_VARIANTS + 1)
+        ctx.set_tile_order(404)
+    src = lgf.Field(ctx, shape, prec)  # copy source, cold by the time it is copied
     src.set_random(99)
-    f = lgf.Field(ctx, SHAPE, prec)
+    f = lgf.Field(ctx, shape, prec)
     f.set_random(1234)
     f.flow(EPS, steps)  # > 1 GB of traffic: evicts src from the 256 MB last-level cache
     f.measure(lgf.MEASURE_FP64 if which != "fp32native" else lgf.MEASURE_DEFAULT)
-    g = lgf.Field(ctx, SHAPE, prec)
+    g = lgf.Field(ctx, shape, prec)
     g.copy_from(src)
     ctx.synchronize()
-    stride = (np.prod(SHAPE) + 63) // 64 * 64
+    stride = (np.prod(shape) + 63) // 64 * 64
     nbytes = 72 * stride * (8 if prec == lgf.FP64 else 4)
-    print(json.dumps({"profile": which, "steps": steps, "copy_kernel_read_bytes": int(nbytes),
-                      "copy_kernel_write_bytes": int(nbytes),
-                      "stage_algorithmic_bytes_per_step": BYTES_PER_LU[prec] * 4 * int(np.prod(SHAPE))}))
+    print(json.dumps({"profile": which, "lattice": list(shape), "steps": steps,
+                      "copy_kernel_read_bytes": int(nbytes), "copy_kernel_write_bytes": int(nbytes),
+                      "stage_algorithmic_bytes_per_step": BYTES_PER_LU[prec] * 4 * int(np.prod(shape))}))
+
+
+def valu_floor(kernel, ms_per_step, sites):
+    """VALU issue floor of an RK3 step of the named FP32 stage kernel, from
+    the committed profiles: waves x VALU instructions per wave x 4 cycles
+    over 1024 SIMDs at the shader clock measured under the same load, and
+    its ratio to the measured step time."""
+    p = os.path.join(ROOT, "profiles", "valu_floor_fp32_stage.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    ipw = d["valu_instr_per_wave"].get(kernel)
+    if not ipw:
+        return None
+    waves = 4 * ((sites + 63) // 64)
+    clock = d["sclk_ghz_under_load"]
+    floor_ms = waves * sum(ipw) * 4 / (1024 * clock * 1e9) * 1e3
+    return {"kernel": kernel, "clock_ghz": clock, "valu_instr_per_wave": ipw, "floor_ms_per_step": floor_ms,
+            "achieved_ms_per_step": ms_per_step, "frac": floor_ms / ms_per_step,
+            "source": "profiles/valu_floor_fp32_stage.json"}
 
 
 def traffic_from_profiles(kernel):
@@ -396,7 +455,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--serve-requests", type=int, default=40, help="0 disables the serving run")
-    ap.add_argument("--profile", choices=["fp32", "fp32pb", "fp64", "fp32native"], default=None,
+    ap.add_argument("--profile", choices=["fp32", "fp32pb", "fp32stream", "fp32hold", "fp64", "fp32native"], default=None,
                     help="run the fixed counter-profiling workload instead of the benchmark")
     args = ap.parse_args()
     if args.profile:
@@ -461,6 +520,8 @@ def main():
                    "measurements": int(len(r["unitarity"]))} for name, r in (("fp32", r32), ("fp64", r64))},
         "tuning": {"stage_variant": {"fp64": r64["stage_variant"], "fp32": r32["stage_variant"]},
                    "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]}},
+        "valu_floor": valu_floor(stage_kernel_name(lgf, r32["stage_variant"]), r32["flow_ms_per_step"],
+                                 int(np.prod(SHAPE))),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic_from_profiles(stage_kernel_name(lgf, r32["stage_variant"])),
