@@ -1,0 +1,48 @@
+"""CPU checks of bench.py's bookkeeping: the stage-kernel names it reports,
+the committed profile lookups behind roofline.traffic and valu_floor, and
+its refusal of more than one GPU."""
+import importlib.util
+import os
+import subprocess
+import sys
+
+from msc_project_amd import lgf
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SITES = 24 ** 4
+
+
+def load_bench():
+    spec = importlib.util.spec_from_file_location("bench_under_test", os.path.join(ROOT, "bench.py"))
+    m = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(m)
+    return m
+
+
+def test_stage_kernel_names_follow_the_variant_groups():
+    b = load_bench()
+    assert b.stage_kernel_name(lgf, 0) == "k_rk3_stage_pl<float>"
+    assert b.stage_kernel_name(lgf, lgf.FP32_COMPACT_VARIANTS - 1) == "k_rk3_stage_pl<float>"
+    assert b.stage_kernel_name(lgf, lgf.FP32_COMPACT_VARIANTS) == "k_rk3_stage_pb<float>"
+    assert b.stage_kernel_name(lgf, lgf.FP32_SHARED_PLAQUETTE_VARIANTS) == "k_rk3_stage<float>"
+
+
+def test_profile_lookups_cover_both_shared_stage_kernels():
+    b = load_bench()
+    algorithmic = 576.0 * 4 * SITES / 3  # bytes per launch: one third of a link-update per link
+    for k in ("k_rk3_stage_pl<float>", "k_rk3_stage_pb<float>"):
+        t = b.traffic_from_profiles(k)
+        assert algorithmic <= t < 3 * algorithmic, (k, t)
+        v = b.valu_floor(k, 0.25, SITES)
+        assert v["kernel"] == k and len(v["valu_instr_per_wave"]) == 3
+        assert 1.5 < v["clock_ghz"] <= 2.4
+        assert 0.15 < v["floor_ms_per_step"] < 0.25 and abs(v["frac"] - v["floor_ms_per_step"] / 0.25) < 1e-12
+    # the per-link kernel is not profiled: no figure rather than another kernel's
+    assert b.traffic_from_profiles("k_rk3_stage<float>") is None
+    assert b.valu_floor("k_rk3_stage<float>", 0.25, SITES) is None
+
+
+def test_more_than_one_gpu_is_refused():
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 2 and "one GPU" in r.stderr
