@@ -36,6 +36,12 @@ void set_checkpoint_every(int n) {
     g_checkpoint_every = n;
 }
 int checkpoint_every() { return g_checkpoint_every; }
+static int g_autotune_steps = 0;
+void set_autotune(int nsteps) {
+    if (nsteps < 0) throw Error(LGF_E_ARG, "autotune step count must be >= 0");
+    g_autotune_steps = nsteps;
+}
+int autotune_steps() { return g_autotune_steps; }
 
 void check(int rc, const char* what) {
     if (rc != LGF_OK) {
@@ -320,6 +326,24 @@ void QdensityDistribution::calculate(double) {
     if (!out) throw Error(LGF_E_IO, "cannot write " + path);
 }
 
+// Tunes the context for `n` lattices of the run's shape in the flow
+// precision, once per (shape, n, precision), before the fields of a driver
+// exist (a batch takes the tile order in force when it is created).
+static void tune_for(SimulationParameters& params, int n) {
+    if (lgf_host::autotune_steps() <= 0 || n <= 0) return;
+    static std::vector<std::vector<int>> done;
+    std::vector<int> key{params.getshape(0), params.getshape(1), params.getshape(2), params.getshape(3), n,
+                         default_precision()};
+    if (std::find(done.begin(), done.end(), key) != done.end()) return;
+    std::vector<int> shapes;
+    for (int k = 0; k < n; ++k)
+        for (int d = 0; d < 4; ++d) shapes.push_back(params.getshape(d));
+    check(lgf_autotune_batch(context(), shapes.data(), n, default_precision(), lgf_host::autotune_steps(), nullptr,
+                             nullptr),
+          "lgf_autotune_batch");
+    done.push_back(key);
+}
+
 // ---------------------------------------------------------------------------
 // LGF/LGF.cpp:163-196
 // Flows `steps` steps; with checkpoints (lgf_host::checkpoint_every) the
@@ -352,6 +376,7 @@ static void flow_with_checkpoints(SimulationParameters& params, FlowParameters& 
 void FlowSavedGaugeConfigurations(SimulationParameters& params, FlowParameters& flowParams) {
     const std::string UpdateMethod = input_method();
     Lattice lattice(params, true);
+    tune_for(params, 1);
     Wilson action(params.getbeta());
     for (int i = flowParams.getConfigurationStart(); i <= flowParams.getConfigurationStop(); i++) {
         SU3_field U(lattice, params.getextdofs());
@@ -385,6 +410,7 @@ void FlowSavedGaugeConfigurationsBatched(SimulationParameters& params, FlowParam
     const int c0 = flowParams.getConfigurationStart(), c1 = flowParams.getConfigurationStop();
     const int n = c1 - c0 + 1;
     if (n <= 0) return;
+    tune_for(params, n);
     std::vector<std::unique_ptr<SU3_field>> U;
     std::vector<lgf_field*> handles;
     for (int i = c0; i <= c1; ++i) {
@@ -435,6 +461,7 @@ void FlowSavedGaugeConfigurationsBatched(SimulationParameters& params, FlowParam
 // LGF/LGF.cpp:197-226
 void ResumeFlowedConfiguration(SimulationParameters& params, FlowParameters& flowParams, double flowTime_pickup) {
     Lattice lattice(params, true);
+    tune_for(params, 1);
     Wilson action(params.getbeta());
     for (int i = flowParams.getConfigurationStart(); i <= flowParams.getConfigurationStop(); i++) {
         SU3_field U(lattice, params.getextdofs());

@@ -49,6 +49,14 @@ int default_precision();
 // clover arithmetic of measurements (LGF_MEASURE_FP64 default)
 void set_measure_flags(int flags);
 int measure_flags();
+// Launch tuning of the flow (0 = off, the default): before the first flow of
+// a lattice shape, the drivers time every tile order and stage variant for
+// `nsteps` RK3 steps on seeded lattices of that shape in the flow precision
+// (lgf_autotune_batch; as many lattices as the batched driver flows at
+// once) and keep the fastest.  FP64 results do not depend on the choice;
+// FP32 results can differ from the untuned ones by FP32 rounding.
+void set_autotune(int nsteps);
+int autotune_steps();
 
 }  // namespace lgf_host
 

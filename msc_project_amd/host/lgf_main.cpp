@@ -11,6 +11,10 @@
 //   --checkpoint-every=N   (flow, resume) every N steps, a multiple of the
 //                      measurement interval, append the observables so far to
 //                      the CSVs and save the configuration as <id>_Flowtime<t>
+//   --autotune[=N]     before flowing, time every tile order and stage variant
+//                      on the run's lattice shape for N RK3 steps (default 5)
+//                      and keep the fastest (launch tuning only; FP64 results
+//                      are unchanged)
 //   --precision=fp64|fp32      flow arithmetic (fp64 default)
 //   --measure=fp64|native      clover arithmetic of measurements (fp64 default)
 //   --input-method=NAME        input update-method directory (default LHMC_3_of_0_330000)
@@ -44,6 +48,8 @@ int main(int argc, char** argv) {
             else if (a == "--measure=native") lgf_host::set_measure_flags(LGF_MEASURE_DEFAULT);
             else if (a.rfind("--input-method=", 0) == 0) lgf_host::set_input_method(val("--input-method="));
             else if (a.rfind("--device=", 0) == 0) lgf_host::set_device(std::stoi(val("--device=")));
+            else if (a == "--autotune") lgf_host::set_autotune(5);
+            else if (a.rfind("--autotune=", 0) == 0) lgf_host::set_autotune(std::stoi(val("--autotune=")));
             else if (a.rfind("--checkpoint-every=", 0) == 0)
                 lgf_host::set_checkpoint_every(std::stoi(val("--checkpoint-every=")));
             else throw lgf_host::Error(LGF_E_ARG, "unknown option " + a);

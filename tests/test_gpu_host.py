@@ -76,6 +76,31 @@ def test_host_flow_config1(tmp_path, mode):
         assert os.path.getsize(flowed) == np.prod(shape) * 576
 
 
+@pytest.mark.parametrize("mode", ["flow", "batch"])
+def test_host_autotune_leaves_fp64_outputs_unchanged(tmp_path, mode):
+    """--autotune only chooses launch parameters (tile order and stage
+    variant, every FP64 choice computes the same links and sums): the CSVs,
+    their full-precision twins and the flowed configurations are byte for
+    byte those of the untuned run"""
+    d, shape, cfgs = config1_inputs()
+    outs = []
+    for tag, opts in (("plain", ()), ("tuned", ("--autotune=2",))):
+        folder = str(tmp_path / tag) + "/"
+        place_inputs(folder, shape, d["beta"], cfgs)
+        run_lgf(folder, shape, d["beta"], len(cfgs) - 1, d["steps"], d["eps"], d["every"], "--mode=" + mode, *opts)
+        files = {}
+        for root, _, names in os.walk(folder):
+            for n in names:
+                path = os.path.join(root, n)
+                if "LHMC_3_of_0_330000" not in path:
+                    with open(path, "rb") as fh:
+                        files[os.path.relpath(path, folder)] = fh.read()
+        outs.append(files)
+    assert len(outs[0]) >= 3 * len(cfgs) and outs[0].keys() == outs[1].keys()
+    for k in outs[0]:
+        assert outs[0][k] == outs[1][k], k
+
+
 def test_host_qdensity_small(tmp_path):
     d, U = small_input()
     shape = tuple(d["shape"])
