@@ -381,9 +381,9 @@ def hold_flow(lgf, ctx, steps):
 
 def profile_run(which, steps):
     """Fixed workload for rocprofv3 counter runs: `steps` RK3 steps of a
-    seeded random 24^4 lattice in one precision (direct launches), one clover
-    measurement, and one plain copy of a link array (k_convert, a kernel of
-    known traffic that calibrates the counters).  "fp32stream" runs the same
+    seeded random 24^4 lattice in one precision (direct launches, tile order
+    404), one clover measurement, and one plain copy of a link array
+    (k_convert, a kernel of known traffic that calibrates the counters).  "fp32stream" runs the same
     on a 32^4 lattice, which does not fit the last-level cache, with the
     stage kernel and tile order the autotuner picks for it (the
     shared-backward-link kernel with non-temporal stores, x0/x2 blocks of 4)."""
@@ -394,11 +394,11 @@ def profile_run(which, steps):
         return
     prec = lgf.FP64 if which == "fp64" else lgf.FP32
     shape = STREAM_SHAPE if which == "fp32stream" else SHAPE
+    ctx.set_tile_order(404)  # the order the autotuner picks for these lattices (x0 and x2 blocks of 4)
     if which == "fp32pb":  # the shared-backward-link stage kernel
         ctx.set_stage_variant(lgf.FP32, lgf.FP32_COMPACT_VARIANTS)
     if which == "fp32stream":
         ctx.set_stage_variant(lgf.FP32, lgf.FP32_COMPACT_VARIANTS + 1)
-        ctx.set_tile_order(404)
     src = lgf.Field(ctx, shape, prec)  # copy source, cold by the time it is copied
     src.set_random(99)
     f = lgf.Field(ctx, shape, prec)
