@@ -198,6 +198,23 @@ def batched(lgf, ctx):
         fields.append(f)
     # tile order and stage variant tuned for this mix of shapes (seeded lattices, before the timed run)
     best, _ = ctx.autotune_batch([s for s, _ in specs], lgf.FP32, nsteps=5)
+    lu = 4.0 * sum(np.prod(s) for s, _ in specs) * 300
+    # the flow alone, on copies (graph-replayed steps)
+    copies = []
+    for f in fields:
+        c = lgf.Field(ctx, f.shape, lgf.FP32)
+        c.copy_from(f)
+        copies.append(c)
+    cb = lgf.Batch(ctx, copies)
+    cb.flow(EPS, 4, use_graph=True)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    cb.flow(EPS, 60, use_graph=True)
+    ctx.synchronize()
+    flow_only = lu / 300 * 60 / (time.perf_counter() - t0)
+    cb.close()
+    for c in copies:
+        c.close()
     b = lgf.Batch(ctx, fields)
     b.flow(EPS, 2, use_graph=True)  # capture
     ctx.synchronize()
@@ -210,11 +227,12 @@ def batched(lgf, ctx):
     b.measure_finish()
     ctx.synchronize()
     wall = time.perf_counter() - t0
-    lu = 4.0 * sum(np.prod(s) for s, _ in specs) * 300
     b.close()
     for f in fields:
         f.close()
     return {"configs": len(specs), "link_updates": lu, "wall_s": wall, "link_updates_per_s": lu / wall,
+            "flow_only_link_updates_per_s": flow_only,
+            "flow_only_hbm_roofline_frac": flow_only * BYTES_PER_LU[1] / (HBM_PEAK_GBS * 1e9),
             "steps": 300, "measure_every": 10, "precision": "fp32",
             "tuning": {"stage_variant": best, "tile_order": ctx.tile_order()}}
 
