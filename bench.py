@@ -24,12 +24,7 @@ import argparse
 import ctypes
 import json
 import os
-import re
-import shutil
-import signal
-import subprocess
 import sys
-import tempfile
 import threading
 import time
 
@@ -46,10 +41,16 @@ FLOW_TIMING_STEPS = 60  # flow-only timing: 180 stage launches between two HIP e
 THERM_SWEEPS = 200
 SEED = 24242424
 HBM_PEAK_GBS = 8000.0
-# algorithmic HBM bytes per link-update: 6 link transfers of 18 reals and 4
-# accumulator transfers of 9 reals (the accumulator is stored as its 9
-# independent reals), (6*18 + 4*9) * sizeof(real)
-BYTES_PER_LU = {0: 1152.0, 1: 576.0}
+# Algorithmic HBM bytes per link-update, the roofline basis: a link is read
+# and written once per stage (6 transfers of 18 reals) and the RK3
+# accumulator is written, read+written and read (4 transfers of a 3x3
+# complex matrix), (6 + 4) * 18 * sizeof(real)
+BYTES_PER_LU = {0: 1440.0, 1: 720.0}
+# the same traffic with the accumulator stored as the 9 independent reals of
+# a traceless anti-Hermitian matrix, as this library stores it:
+# (6*18 + 4*9) * sizeof(real)
+STORED_BYTES_PER_LU = {0: 1152.0, 1: 576.0}
+ACCURACY_STEPS = 500  # the accuracy pass always flows to t = 5, past t0 (t^2 E = 0.3)
 
 
 class HipEvents:
@@ -137,26 +138,42 @@ def run_job(lgf, ctx, src, prec, steps, warmup, ev):
     wall = time.perf_counter() - t0
     b.close()
     f.close()
-    # untimed replay of the same trajectory (the flow is deterministic): the
-    # deviation from unitarity, max and mean of |U^dag U - 1| over the links,
-    # at every measurement time
+    t = np.array([0.0] + [EPS * k for k in range(MEASURE_EVERY, steps + 1, MEASURE_EVERY)])[: len(E)]
+    return {"wall_s": wall, "flow_ms_per_step": flow_ms, "measure_ms": meas_ms, "stage_variant": variant,
+            "tile_order": ctx.tile_order(), "autotune_ms_per_step": tune_ms, "t": t, "E": np.array(E),
+            "Q": np.array(Q)}
+
+
+def accuracy_run(lgf, ctx, src, prec, steps=ACCURACY_STEPS):
+    """Untimed flow of a copy of `src` for `steps` RK3 steps, E and Q every
+    MEASURE_EVERY steps (final sweep in FP64, as in the timed job): the
+    series the FP32-vs-FP64 accuracy is judged on, with the deviation from
+    unitarity (max and mean of ||U^dag U - 1|| over the links) at every
+    measurement."""
+    flags = lgf.MEASURE_DEFAULT
     f = lgf.Field(ctx, SHAPE, prec)
     f.copy_from(src)
     b = lgf.Batch(ctx, [f])
-    unit = [f.diagnostics()[1:]]
+    E, Q, unit = [], [], []
+
+    def measure(fl):
+        o = b.observables(fl)[0]
+        E.append(o[0]), Q.append(o[1])
+        unit.append(f.diagnostics()[1:])
+
+    measure(flags)
     done = 0
     while done < steps:
         n = min(MEASURE_EVERY, steps - done)
         b.flow(EPS, n, use_graph=True)
         done += n
-        unit.append(f.diagnostics()[1:])
+        measure(flags if done < steps else lgf.MEASURE_FP64)
     b.close()
     f.close()
+    t = np.array([0.0] + [EPS * k for k in range(MEASURE_EVERY, steps + 1, MEASURE_EVERY)])
     unit = np.array(unit)
-    t = np.array([0.0] + [EPS * k for k in range(MEASURE_EVERY, steps + 1, MEASURE_EVERY)])[: len(E)]
-    return {"wall_s": wall, "flow_ms_per_step": flow_ms, "measure_ms": meas_ms, "stage_variant": variant,
-            "tile_order": ctx.tile_order(), "autotune_ms_per_step": tune_ms, "t": t, "E": np.array(E),
-            "Q": np.array(Q), "unitarity_max": float(unit[:, 0].max()), "unitarity": unit}
+    return {"t": t, "E": np.array(E), "Q": np.array(Q), "unitarity": unit,
+            "unitarity_max": float(unit[:, 0].max())}
 
 
 def t0_of(t, E):
@@ -233,6 +250,12 @@ def batched(lgf, ctx):
     return {"configs": len(specs), "link_updates": lu, "wall_s": wall, "link_updates_per_s": lu / wall,
             "flow_only_link_updates_per_s": flow_only,
             "flow_only_hbm_roofline_frac": flow_only * BYTES_PER_LU[1] / (HBM_PEAK_GBS * 1e9),
+            "roofline": {"bound": "hbm", "achieved": flow_only * BYTES_PER_LU[1] / 1e9, "peak": HBM_PEAK_GBS,
+                         "unit": "GB/s", "frac": flow_only * BYTES_PER_LU[1] / (HBM_PEAK_GBS * 1e9),
+                         "frac_stored_bytes": flow_only * STORED_BYTES_PER_LU[1] / (HBM_PEAK_GBS * 1e9),
+                         "basis": "720 B per link-update (FP32 links and accumulator); frac_stored_bytes: "
+                                  "576 B, the accumulator as 9 reals",
+                         "time": "wall clock of 60 graph-replayed RK3 steps of the whole batch"},
             "steps": 300, "measure_every": 10, "precision": "fp32",
             "tuning": {"stage_variant": best, "tile_order": ctx.tile_order()}}
 
@@ -303,13 +326,25 @@ def stage_kernel_name(lgf, variant):
     return "k_rk3_stage<float>"
 
 
-def cpu_baseline(budget_s=20.0):
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(budget_s=20.0, threads=None):
     """The CPU oracle (a C port of the reference path, OpenMP over sites) on
     whole RK3 steps of a 24^4 lattice (seeded Haar-random links; the flow's
-    CPU cost does not depend on the values), within a time budget."""
+    CPU cost does not depend on the values), for about budget_s / 2 seconds."""
     from oracle import oracle as O
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+    threads = threads or int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
     os.environ["OMP_NUM_THREADS"] = str(threads)
+    O.set_threads(threads)
     V = O.generate(SHAPE, BETA, seed=1234, start="hot", sweeps=0)
     t0 = time.perf_counter()
     n = 0
@@ -320,54 +355,30 @@ def cpu_baseline(budget_s=20.0):
             break
     dt = time.perf_counter() - t0
     return {"value": 4.0 * np.prod(SHAPE) * n / dt, "unit": "link-updates/s", "cores": threads, "kind": "port",
+            "host": {"cpu": cpu_model(), "logical_cpus": os.cpu_count()},
             "sample": "%d RK3 steps of a 24^4 lattice with the CPU oracle (oracle/lgf_oracle.c, %d OpenMP "
                       "threads), no measurements, %.1f s" % (n, threads, dt)}
 
 
-def cpu_baseline_reference(U, shape=SHAPE, ranks=16, steps=(1, 5), timeout_s=300):
-    """The reference program itself (built from its own sources by
-    oracle/ref/Makefile into oracle/_ref/lgf_ref) on the same input, as
-    `ranks` MPI processes (cuts 1,1,1,1: a 2x2x2x2 decomposition).  Its
-    FlowSavedGaugeConfigurations measures at t = 0 whatever the interval, so
-    the difference between a 5-step and a 1-step run (interval larger than
-    the run) is four RK3 steps.  None when the program or MPI is missing."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "lgf_ref")
-    mpiexec = "/opt/conda/bin/mpiexec"
-    if not (os.path.exists(exe) and os.path.exists(mpiexec)):
+def reference_record():
+    """The reference program's own CPU throughput on a 24^4 lattice, from the
+    committed record that oracle/ref/time_reference.py writes in the build
+    container (the reference does not travel to the GPU host): flow-only and
+    job rates at 1 and 8 MPI ranks, and the CPU oracle timed on that host, the
+    bridge to this host's cpu_baseline."""
+    p = os.path.join(ROOT, "profiles", "cpu_reference_flow_24x24x24x24.json")
+    if not os.path.exists(p):
         return None
-    folder = tempfile.mkdtemp(prefix="lgf_refbase_") + "/"
-    try:
-        d = os.path.join(folder, "ensembles", "su3", "beta" + ("%f" % BETA).replace(".", "_"),
-                         "X".join(map(str, shape)), "LHMC_3_of_0_330000")
-        os.makedirs(d)
-        np.ascontiguousarray(U, dtype=np.float64).tofile(os.path.join(d, "torus_extdof4_0.bin"))
-        walls = []
-        for n in steps:
-            args = [mpiexec, "-n", str(ranks), exe, "flow", "4", "4", repr(BETA), *map(str, shape),
-                    "1", "1", "1", "1", "0", "0", str(n), repr(EPS), "100000", folder]
-            proc = subprocess.Popen(args, stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True,
-                                    start_new_session=True)
-            try:
-                _, err = proc.communicate(timeout=timeout_s)
-            except subprocess.TimeoutExpired:
-                os.killpg(proc.pid, signal.SIGKILL)  # mpiexec and its ranks
-                proc.communicate()
-                return None
-            m = re.search(r"lgf_ref wall_s (\S+) ranks (\d+)", err or "")
-            if proc.returncode != 0 or not m:
-                return None
-            walls.append(float(m.group(1)))
-        per_step = (walls[1] - walls[0]) / (steps[1] - steps[0])
-        if per_step <= 0:
-            return None
-        return {"value": 4.0 * np.prod(shape) / per_step, "unit": "link-updates/s", "cores": ranks,
-                "kind": "reference",
-                "sample": "the reference's FlowSavedGaugeConfigurations on the same %s input, %d MPI ranks "
-                          "(cuts 1,1,1,1), difference of a %d-step and a %d-step run (%.1f s and %.1f s, t = 0 "
-                          "measurement in both)" % ("x".join(map(str, shape)), ranks, steps[1], steps[0], walls[1],
-                                                    walls[0])}
-    finally:
-        shutil.rmtree(folder, ignore_errors=True)
+    with open(p) as f:
+        d = json.load(f)
+    out = {"kind": "reference", "unit": "link-updates/s", "host": d["host"], "source": os.path.relpath(p, ROOT),
+           "timed_where": "build container (the reference program is not run on the GPU host)"}
+    for ranks, r in sorted(d["runs"].items(), key=lambda kv: int(kv[0])):
+        out["ranks_%s" % ranks] = {"cores": r["cores"], "flow_only": r["flow_only_link_updates_per_s"],
+                                   "job_measure_every_5": r["job_link_updates_per_s_measure_every_5"]}
+    if "oracle_port" in d:
+        out["oracle_port_same_host"] = d["oracle_port"]
+    return out
 
 
 STREAM_SHAPE = (32, 32, 32, 32)  # FP32 working set 604 MB: streams from HBM
@@ -472,7 +483,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
-    ap.add_argument("--serve-requests", type=int, default=40, help="0 disables the serving run")
+    ap.add_argument("--serve-requests", type=int, default=500,
+                    help="requests of the serving stream (shortest-first policy); 0 disables it")
+    ap.add_argument("--compare-requests", type=int, default=40,
+                    help="requests of the comparison stream (every job steps); 0 disables it")
     ap.add_argument("--profile", choices=["fp32", "fp32pb", "fp32stream", "fp32hold", "fp64", "fp32native"], default=None,
                     help="run the fixed counter-profiling workload instead of the benchmark")
     args = ap.parse_args()
@@ -490,24 +504,36 @@ def main():
     src = thermalised(lgf, ctx, SHAPE, BETA, SEED, THERM_SWEEPS)
     ctx.synchronize()
     plaq = src.diagnostics()[0]
-    U_in = src.download() if not args.no_cpu else None  # the reference CPU baseline's input
     log("input: 24^4 beta 6.0, %d HB/OR iterations in %.1f s, <P> = %.5f" % (THERM_SWEEPS, time.perf_counter() - tg,
                                                                              plaq))
-    res = {}
+    res, accr = {}, {}
     for prec in (lgf.FP64, lgf.FP32):
+        name = "fp64" if prec == lgf.FP64 else "fp32"
         res[prec] = run_job(lgf, ctx, src, prec, args.steps, args.warmup, ev)
         r = res[prec]
         log("%s: job %.3f s, flow %.4f ms/step, measure %.3f ms, variant %d, tile order %d" % (
-            "fp64" if prec == 0 else "fp32", r["wall_s"], r["flow_ms_per_step"], r["measure_ms"], r["stage_variant"],
-            r["tile_order"]))
-        log("%s: |U^dag U - 1| max/mean at t = 0, 0.05, ...: %s" % (
-            "fp64" if prec == 0 else "fp32", " ".join("%.2e/%.2e" % (a, m) for a, m in r["unitarity"])))
+            name, r["wall_s"], r["flow_ms_per_step"], r["measure_ms"], r["stage_variant"], r["tile_order"]))
+        # untimed accuracy pass to t = 5 with the same tuned kernels
+        accr[prec] = accuracy_run(lgf, ctx, src, prec)
+        log("%s: |U^dag U - 1| max/mean at t = 0, 0.05, ..., 5: %s" % (
+            name, " ".join("%.2e/%.2e" % (a, m) for a, m in accr[prec]["unitarity"])))
     lu_step = 4.0 * np.prod(SHAPE)
     r32, r64 = res[lgf.FP32], res[lgf.FP64]
-    acc = accuracy(r64, r32)
+    a32, a64 = accr[lgf.FP32], accr[lgf.FP64]
+    acc = accuracy(a64, a32)
+    acc["pass"] = "untimed flow of both precisions from the same input to t = %g (%d steps), E and Q every %d " \
+                  "steps" % (ACCURACY_STEPS * EPS, ACCURACY_STEPS, MEASURE_EVERY)
+    timed = accuracy(r64, r32)
+    acc["timed_job"] = {k: timed[k] for k in ("max_abs_dQ", "max_rel_dt2E", "measurements")}
     log("accuracy: %s" % json.dumps(acc))
     value = lu_step * args.steps / r32["wall_s"]
-    achieved = BYTES_PER_LU[1] * lu_step / (r32["flow_ms_per_step"] * 1e-3) / 1e9
+    kern = stage_kernel_name(lgf, r32["stage_variant"])
+    # one launch of a stage kernel advances every link of the lattice by one of
+    # the three stages: 4V/3 link-updates; its mean duration from the HIP
+    # events around FLOW_TIMING_STEPS graph-replayed steps on the library's stream
+    launch_s = r32["flow_ms_per_step"] * 1e-3 / 3
+    lu_launch = lu_step / 3
+    achieved = BYTES_PER_LU[1] * lu_launch / launch_s / 1e9
     line = {
         "metric": "RK3 flow link-updates/s (FP32 arithmetic, FP64 reductions) on one 24^4 lattice",
         "value": value,
@@ -522,8 +548,10 @@ def main():
         "dtype": "fp32",
         "data": "synthetic: 24^4 beta 6.0 from a seeded cold start and 200 GPU heatbath/OR iterations "
                 "(<P> = %.5f)" % plaq,
-        "config": {"workload": "single 24^4 lattice, beta 6.0, RK3 eps 0.01, 500 steps (t=5), E/t^2E/Q every 5 steps",
-                   "lattice": list(SHAPE), "eps": EPS, "measure_every": MEASURE_EVERY},
+        "config": {"workload": "single 24^4 lattice, beta 6.0, RK3 eps 0.01, E/t^2E/Q every 5 steps: %d steps timed "
+                               "(t = %g); accuracy pass to t = %g" % (args.steps, args.steps * EPS,
+                                                                      ACCURACY_STEPS * EPS),
+                   "lattice": list(SHAPE), "eps": EPS, "measure_every": MEASURE_EVERY, "timed_steps": args.steps},
         "fp64_lups": lu_step * args.steps / r64["wall_s"],
         "fp32_over_fp64_job": r64["wall_s"] / r32["wall_s"],
         "fp32_flow_only_lups": lu_step / (r32["flow_ms_per_step"] * 1e-3),
@@ -531,20 +559,22 @@ def main():
         "fp32_over_fp64_flow": r64["flow_ms_per_step"] / r32["flow_ms_per_step"],
         "measure_ms": {"fp32": r32["measure_ms"], "fp64": r64["measure_ms"]},
         "fp32_vs_fp64": acc,
-        "unitarity_max": {"fp32": r32["unitarity_max"], "fp64": r64["unitarity_max"]},
+        "unitarity_max": {"fp32": a32["unitarity_max"], "fp64": a64["unitarity_max"]},
         "unitarity_every_measurement": {
             name: {"max_of_max": float(r["unitarity"][:, 0].max()), "max_of_mean": float(r["unitarity"][:, 1].max()),
                    "final_max": float(r["unitarity"][-1, 0]), "final_mean": float(r["unitarity"][-1, 1]),
-                   "measurements": int(len(r["unitarity"]))} for name, r in (("fp32", r32), ("fp64", r64))},
+                   "measurements": int(len(r["unitarity"]))} for name, r in (("fp32", a32), ("fp64", a64))},
         "tuning": {"stage_variant": {"fp64": r64["stage_variant"], "fp32": r32["stage_variant"]},
                    "tile_order": {"fp64": r64["tile_order"], "fp32": r32["tile_order"]}},
-        "valu_floor": valu_floor(stage_kernel_name(lgf, r32["stage_variant"]), r32["flow_ms_per_step"],
-                                 int(np.prod(SHAPE))),
+        "valu_floor": valu_floor(kern, r32["flow_ms_per_step"], int(np.prod(SHAPE))),
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic_from_profiles(stage_kernel_name(lgf, r32["stage_variant"])),
-                     "kernel": "%s (stage variant %d, 3 launches per RK3 step, 576 B per link-update algorithmic)" % (
-                         stage_kernel_name(lgf, r32["stage_variant"]), r32["stage_variant"])},
+                     "frac_stored_bytes": STORED_BYTES_PER_LU[1] * lu_launch / launch_s / 1e9 / HBM_PEAK_GBS,
+                     "traffic": traffic_from_profiles(kern),
+                     "launch_us": launch_s * 1e6,
+                     "kernel": "%s (stage variant %d, 3 launches per RK3 step)" % (kern, r32["stage_variant"]),
+                     "basis": "720 B per link-update (FP32 links and accumulator) x 4V/3 link-updates per launch; "
+                              "frac_stored_bytes: 576 B, the accumulator stored as its 9 independent reals"},
     }
     if not args.no_batch:
         line["batched_16_uneven"] = batched(lgf, ctx)
@@ -555,16 +585,16 @@ def main():
         line["serving"] = serving(lgf, args.serve_requests, value, policy=lgf.SERVE_SHORTEST_FIRST, age_rate=value)
         line["p99_latency_s"] = line["serving"]["p99_latency_s"]
         log("serving: %s" % json.dumps(line["serving"]))
-        # the same request stream with every admitted job stepping each round
-        line["serving_policy_all"] = serving(lgf, args.serve_requests, value, policy=lgf.SERVE_ALL)
+    if args.compare_requests > 0:
+        # a shorter stream of the same mix with every admitted job stepping each round
+        line["serving_policy_all"] = serving(lgf, args.compare_requests, value, policy=lgf.SERVE_ALL)
         log("serving (all jobs step): %s" % json.dumps(line["serving_policy_all"]))
     if not args.no_cpu:
-        port = cpu_baseline()
-        ref = cpu_baseline_reference(U_in)
-        line["cpu_baseline"] = ref if ref else port
-        if ref:
-            line["cpu_baseline_port"] = port
+        line["cpu_baseline"] = cpu_baseline()
         log("cpu baseline: %s" % json.dumps(line["cpu_baseline"]))
+        ref = reference_record()
+        if ref:
+            line["cpu_baseline_reference"] = ref
     print(json.dumps(line), flush=True)
 
 

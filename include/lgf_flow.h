@@ -144,6 +144,14 @@ int lgf_field_shape(const lgf_field* f, int shape[4], int* precision);
 /* links: volume*4*18 doubles in the reference's saved-configuration order */
 int lgf_field_upload(lgf_field* f, const double* links);
 int lgf_field_download(lgf_field* f, double* links);
+/* flow time of the field: advanced by eps at every RK3 step it takes, as
+ * GradientFlow::flow accumulates it (LGF/Gradient_Flow/GradientFlow.cpp:
+ * 52-58); 0 at creation.  Set it as GradientFlow::SetFlowTime does
+ * (GradientFlow.cpp:64-66) when a flow is resumed from a saved
+ * configuration.  It labels measurements (lgf_batch_track_t0); the flow
+ * itself does not depend on it. */
+int lgf_field_flow_time(const lgf_field* f, double* t);
+int lgf_field_set_flow_time(lgf_field* f, double t);
 int lgf_field_set_cold(lgf_field* f);
 /* seeded Haar-random SU(3) links (a "hot start"; same Philox4x32-10 counters
  * as the CPU oracle's, values equal to rounding) */
@@ -203,6 +211,19 @@ int lgf_batch_measure_observables(lgf_batch* b, int flags, double* obs);
  * (LGF/Gradient_Flow/GradientFlow.cpp:52-58). */
 int lgf_batch_measure_start(lgf_batch* b, int flags);
 int lgf_batch_measure_finish(lgf_batch* b, double* obs);
+/* On-device flow-scale tracking, the t0 of lgf_flow_scale_t0 kept on the
+ * GPU: after lgf_batch_track_t0, every measurement of the batch (synchronous
+ * or deferred) also updates, in stream order and without synchronising,
+ * each field's t0 = the first flow time at which t^2 E reaches `target`,
+ * by linear interpolation of t^2 E between that measurement and the
+ * previous one, t being the field's flow time (lgf_field_flow_time) when
+ * the measurement was enqueued.  lgf_batch_t0 waits for the stream and
+ * returns t0 per field, NaN for a field that has not reached the target.
+ * Calling lgf_batch_track_t0 again restarts the tracking.  The reference
+ * finds t0 in post-processing (DataAnalysis_and_Plotting/t0_extrapolation.py,
+ * t^2 E(t0) = 0.3, doc/thesis.tex:1214-1216). */
+int lgf_batch_track_t0(lgf_batch* b, double target);
+int lgf_batch_t0(lgf_batch* b, double* t0);
 int lgf_measure_start(lgf_field* f, int flags);
 int lgf_measure_finish(lgf_field* f, double* obs);
 
@@ -237,6 +258,10 @@ typedef struct {
  * max_active: jobs flowing at once (0 = unlimited within the budget) */
 int lgf_server_create(int device, size_t budget_bytes, int max_active, lgf_server** out);
 int lgf_server_destroy(lgf_server* s);
+/* LGF_E_ARG for a request that would record more than 1024 measurements
+ * (the capacity of lgf_result): 1 at t = 0, one per multiple of
+ * measure_every up to nsteps, and one at the end when nsteps is not such a
+ * multiple (measure_every = 0: t = 0 and the end) */
 int lgf_server_submit(lgf_server* s, const lgf_request* req, uint64_t* id);
 /* wait (timeout_s < 0: forever) for the job; LGF_E_STATE on timeout */
 int lgf_server_wait(lgf_server* s, uint64_t id, double timeout_s, lgf_result* out);
@@ -258,6 +283,12 @@ int lgf_server_set_policy(lgf_server* s, int policy, int64_t min_sites, double a
  * called concurrently with lgf_server_destroy. */
 int lgf_server_autotune(lgf_server* s, const int* shapes, int nshapes, int precision, int nsteps, int* best);
 int lgf_server_now(lgf_server* s, double* now_s);
+/* Fault injection for tests of the scheduler's error path (no counterpart in
+ * the reference): the next `count` batch creations of the scheduler fail as
+ * if the device were out of memory.  The jobs that were to step in them then
+ * finish with status LGF_E_NOMEM, keep the measurements recorded before the
+ * failure, and free their memory; other jobs are not affected. */
+int lgf_server_inject_batch_failures(lgf_server* s, int count);
 /* counters: jobs done, steps executed, link-updates executed, busy seconds */
 int lgf_server_stats(lgf_server* s, uint64_t* jobs_done, uint64_t* batch_steps, double* link_updates,
                      double* busy_s);

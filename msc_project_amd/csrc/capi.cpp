@@ -226,6 +226,18 @@ extern "C" int lgf_field_shape(const lgf_field* f, int shape[4], int* precision)
     return LGF_OK;
 }
 
+extern "C" int lgf_field_flow_time(const lgf_field* f, double* t) {
+    if (!f || !t) return LGF_E_ARG;
+    *t = f->flow_t;
+    return LGF_OK;
+}
+
+extern "C" int lgf_field_set_flow_time(lgf_field* f, double t) {
+    if (!f || !(t == t)) return LGF_E_ARG;
+    f->flow_t = t;
+    return LGF_OK;
+}
+
 extern "C" int lgf_field_set_cold(lgf_field* f) {
     if (!f) return LGF_E_ARG;
     lgf_ctx* ctx = f->ctx;
@@ -560,6 +572,7 @@ extern "C" int lgf_batch_destroy(lgf_batch* b) {
     if (b->d_partials_m) (void)hipFree(b->d_partials_m);
     if (b->d_out_m) (void)hipFree(b->d_out_m);
     if (b->h_out_m) (void)hipHostFree(b->h_out_m);
+    if (b->d_t0) (void)hipFree(b->d_t0);
     if (b->ev_done) (void)hipEventDestroy(b->ev_done);
     delete b;
     return LGF_OK;
@@ -600,6 +613,13 @@ static int enqueue_step(lgf_batch* b, int ping) {
     return LGF_OK;
 }
 
+// t += eps once per step, as the reference accumulates its flow time
+// (LGF/Gradient_Flow/GradientFlow.cpp:52-58)
+static void add_flow_time(lgf_batch* b, const double* eps, int steps) {
+    for (size_t i = 0; i < b->fields.size(); ++i)
+        for (int k = 0; k < steps; ++k) b->fields[i]->flow_t += eps[i];
+}
+
 static void advance(lgf_batch* b, int stages) {
     if (stages & 1) {
         b->ping ^= 1;
@@ -635,11 +655,13 @@ extern "C" int lgf_batch_flow_steps(lgf_batch* b, const double* eps, int nsteps,
             }
             HIPTRY(ctx, hipGraphLaunch(b->graph[g], ctx->stream));
             advance(b, 6);
+            add_flow_time(b, eps, 2);
             nsteps -= 2;
         } else {
             rc = enqueue_step(b, b->ping);
             if (rc) return rc;
             advance(b, 3);
+            add_flow_time(b, eps, 1);
             nsteps -= 1;
         }
     }
@@ -703,21 +725,73 @@ static int grow_buffer(lgf_ctx* ctx, void** buf, size_t* have, size_t need, hipS
     return LGF_OK;
 }
 
+// On-device t0 tracking after a measurement whose per-field sums are in
+// `out`: the fields' flow times travel as kernel arguments (captured at
+// enqueue), kT0Chunk fields per launch.
+static int enqueue_t0_track(lgf_batch* b, const double* out) {
+    if (!b->track_t0) return LGF_OK;
+    lgf_ctx* ctx = b->ctx;
+    const int n = (int)b->fields.size();
+    for (int first = 0; first < n; first += kT0Chunk) {
+        T0Args a;
+        a.first = first;
+        a.n = std::min(kT0Chunk, n - first);
+        a.target = b->t0_target;
+        for (int k = 0; k < a.n; ++k) a.t[k] = b->fields[first + k]->flow_t;
+        HIPTRY(ctx, launch_t0_track(b->d_desc, out, a, b->d_t0, ctx->stream));
+    }
+    return LGF_OK;
+}
+
+extern "C" int lgf_batch_track_t0(lgf_batch* b, double target) {
+    if (!b || !(target > 0.0)) return LGF_E_ARG;
+    lgf_ctx* ctx = b->ctx;
+    if (b->pending) return lgf_fail(ctx, LGF_E_STATE, "a measurement of this batch is still outstanding");
+    if (!b->d_t0) {
+        hipError_t e = hipMalloc(&b->d_t0, sizeof(double) * 3 * b->fields.size());
+        if (e != hipSuccess) return lgf_hip_fail(ctx, e, "t0 records");
+    }
+    HIPTRY(ctx, launch_t0_reset(b->d_t0, (int)b->fields.size(), ctx->stream));
+    b->t0_target = target;
+    b->track_t0 = true;
+    return LGF_OK;
+}
+
+extern "C" int lgf_batch_t0(lgf_batch* b, double* t0) {
+    if (!b || !t0) return LGF_E_ARG;
+    lgf_ctx* ctx = b->ctx;
+    if (!b->track_t0) return lgf_fail(ctx, LGF_E_STATE, "t0 is not tracked on this batch (lgf_batch_track_t0)");
+    std::vector<double> rec(3 * b->fields.size());
+    HIPTRY(ctx, hipMemcpyAsync(rec.data(), b->d_t0, sizeof(double) * rec.size(), hipMemcpyDeviceToHost, ctx->stream));
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < b->fields.size(); ++i) t0[i] = rec[3 * i + 2];
+    return LGF_OK;
+}
+
+// one field's reduction record (the sums of k_reduce) -> lgf_observable values
+static bool sums_to_observables(const double* h, int vol, double* o) {
+    const double rq = 1.0 / (4.0 * kPi * kPi);  // LGF/Observables/TopologicalCharge.cpp:21
+    const double rv = 1.0 / (double)vol;
+    o[LGF_OBS_E] = h[0] * rv;
+    o[LGF_OBS_Q] = h[1] * rq;
+    o[LGF_OBS_QABS] = h[2] * rq;
+    o[LGF_OBS_E_IMPROVED] = h[3] * rv;
+    o[LGF_OBS_Q_UNIMPROVED] = h[4] * rq;
+    bool finite = true;
+    for (int k = 0; k < LGF_NUM_OBS; ++k) finite = finite && std::isfinite(o[k]);
+    return finite;
+}
+
+int observables_of_sums(lgf_ctx* ctx, const double* h, int vol, double* obs) {
+    if (!sums_to_observables(h, vol, obs)) return lgf_fail(ctx, LGF_E_NAN, "non-finite energy density or charge");
+    return LGF_OK;
+}
+
 // per-field reduction records -> lgf_observable values
 static int observables_from(lgf_batch* b, const double* h_out, double* obs) {
     bool finite = true;
-    const double rq = 1.0 / (4.0 * kPi * kPi);  // LGF/Observables/TopologicalCharge.cpp:21
-    for (size_t i = 0; i < b->fields.size(); ++i) {
-        const double rv = 1.0 / (double)b->fields[i]->vol;
-        const double* h = h_out + kOutStride * i;
-        double* o = obs + LGF_NUM_OBS * i;
-        o[LGF_OBS_E] = h[0] * rv;
-        o[LGF_OBS_Q] = h[1] * rq;
-        o[LGF_OBS_QABS] = h[2] * rq;
-        o[LGF_OBS_E_IMPROVED] = h[3] * rv;
-        o[LGF_OBS_Q_UNIMPROVED] = h[4] * rq;
-        for (int k = 0; k < LGF_NUM_OBS; ++k) finite = finite && std::isfinite(o[k]);
-    }
+    for (size_t i = 0; i < b->fields.size(); ++i)
+        finite = sums_to_observables(h_out + kOutStride * i, b->fields[i]->vol, obs + LGF_NUM_OBS * i) && finite;
     if (!finite) return lgf_fail(b->ctx, LGF_E_NAN, "non-finite energy density or charge");
     return LGF_OK;
 }
@@ -737,6 +811,8 @@ extern "C" int lgf_batch_measure_observables(lgf_batch* b, int flags, double* ob
         HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, c.first, c.second, b->ping, ctx->lines,
                                   b->d_partials, ctx->stream));
     HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials, kCloverParts, kCloverParts, b->d_out, ctx->stream));
+    rc = enqueue_t0_track(b, b->d_out);
+    if (rc) return rc;
     HIPTRY(ctx, hipMemcpyAsync(b->h_out, b->d_out, sizeof(double) * kOutStride * n, hipMemcpyDeviceToHost,
                                ctx->stream));
     HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
@@ -772,6 +848,8 @@ extern "C" int lgf_batch_measure_start(lgf_batch* b, int flags) {
         HIPTRY(ctx, launch_clover(b->prec, compute, b->d_desc, b->d_tiles, c.first, c.second, b->ping, ctx->lines,
                                   b->d_partials_m, ctx->stream));
     HIPTRY(ctx, launch_reduce(b->d_desc, n, b->d_partials_m, kCloverParts, kCloverParts, b->d_out_m, ctx->stream));
+    rc = enqueue_t0_track(b, b->d_out_m);
+    if (rc) return rc;
     HIPTRY(ctx, hipMemcpyAsync(b->h_out_m, b->d_out_m, sizeof(double) * kOutStride * n, hipMemcpyDeviceToHost,
                                ctx->stream));
     HIPTRY(ctx, hipEventRecord(b->ev_done, ctx->stream));

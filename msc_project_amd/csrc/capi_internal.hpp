@@ -38,6 +38,7 @@ struct lgf_field {
     int cur = 0;           // links[cur] holds the current configuration
     lgf_batch* solo = nullptr;
     int in_batches = 0;    // user batches that reference this field
+    double flow_t = 0.0;   // flow time, t += eps per RK3 step (GradientFlow's m_flowTime)
 };
 
 struct lgf_batch {
@@ -62,9 +63,17 @@ struct lgf_batch {
     double* d_out_m = nullptr;
     double* h_out_m = nullptr;
     bool pending = false;
+    // on-device flow-scale tracking (lgf_batch_track_t0): per field
+    // {t, t^2 E} of the previous measurement and t0 (NaN until reached)
+    double* d_t0 = nullptr;
+    double t0_target = 0.0;
+    bool track_t0 = false;
 };
 
 int lgf_fail(lgf_ctx* ctx, int code, const std::string& msg);
 int lgf_hip_fail(lgf_ctx* ctx, hipError_t e, const char* what);
 int batch_sync_desc(lgf_batch* b, const double* eps = nullptr);
 int ctx_staging(lgf_ctx* ctx, void** dev, void** host);
+// one field's clover sums (k_reduce record) -> obs[LGF_NUM_OBS]; LGF_E_NAN if
+// any is not finite
+int observables_of_sums(lgf_ctx* ctx, const double* h, int vol, double* obs);

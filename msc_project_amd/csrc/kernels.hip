@@ -895,6 +895,34 @@ __global__ __launch_bounds__(kReduceThreads) void k_reduce(const FieldDesc* __re
 }
 
 // ---------------------------------------------------------------------------
+// flow-scale t0 on the device
+// ---------------------------------------------------------------------------
+// The same interpolation as lgf_flow_scale_t0 (the reference leaves t0 to its
+// analysis scripts, DataAnalysis_and_Plotting/t0_extrapolation.py; t0 is
+// defined by t^2 E(t0) = 0.3, doc/thesis.tex:1214-1216), applied one
+// measurement at a time: E = sum e / V from the reduction that just ran.
+__global__ __launch_bounds__(64) void k_t0_track(const FieldDesc* __restrict__ fds, const double* __restrict__ out,
+                                                 T0Args a, double* __restrict__ rec) {
+    const int k = threadIdx.x;
+    if (k >= a.n) return;
+    const int i = a.first + k;
+    double* r = rec + 3 * i;
+    const double t = a.t[k];
+    const double y = t * t * (out[kOutStride * i] / (double)fds[i].vol);
+    const double tp = r[0], yp = r[1];
+    // a < 0 <= b with a, b the distances of the two t^2 E from the target
+    if (isnan(r[2]) && !isnan(yp) && t > tp && yp - a.target < 0.0 && y - a.target >= 0.0)
+        r[2] = tp + (t - tp) * (a.target - yp) / (y - yp);
+    r[0] = t;
+    r[1] = y;
+}
+
+__global__ __launch_bounds__(256) void k_t0_reset(double* __restrict__ rec, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 3 * n) rec[i] = __builtin_nan("");
+}
+
+// ---------------------------------------------------------------------------
 // diagnostics
 // ---------------------------------------------------------------------------
 // average plaquette (LGF/Field/SU3_field.cpp:485-499) partials: sum over the
@@ -1149,6 +1177,16 @@ hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int nt
         hipLaunchKernelGGL((k_diag<float>), dim3(ntiles), dim3(64), 0, stream, fds, tiles, ping, partials);
     hipLaunchKernelGGL(k_reduce, dim3(nfields), dim3(kReduceThreads), 0, stream, fds, partials, 3, 2, out);
     hipLaunchKernelGGL(k_reduce_max, dim3(nfields), dim3(256), 0, stream, fds, partials, 3, 2, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_t0_track(const FieldDesc* fds, const double* out, const T0Args& a, double* rec, hipStream_t stream) {
+    hipLaunchKernelGGL(k_t0_track, dim3(1), dim3(64), 0, stream, fds, out, a, rec);
+    return hipGetLastError();
+}
+
+hipError_t launch_t0_reset(double* rec, int nfields, hipStream_t stream) {
+    hipLaunchKernelGGL(k_t0_reset, dim3((3 * nfields + 255) / 256), dim3(256), 0, stream, rec, nfields);
     return hipGetLastError();
 }
 

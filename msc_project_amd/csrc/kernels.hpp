@@ -28,6 +28,19 @@ hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2
 // 0..ncomp-1) into out[kOutStride * field + component]
 hipError_t launch_reduce(const FieldDesc* fds, int nfields, const double* partials, int pstride, int ncomp, double* out,
                          hipStream_t stream);
+// On-device t0 tracking: for each field of a batch, after a measurement
+// whose per-field sums are in `out` (kOutStride doubles per field, E sum
+// first), with flow times t[i]: t0 = the linear interpolation of t^2 E = target
+// between the previous and this measurement, at the first crossing.  rec holds
+// 3 doubles per field: previous t, previous t^2 E, t0 (NaN = none yet).
+constexpr int kT0Chunk = 32;  // fields per launch (their flow times travel as kernel arguments)
+struct T0Args {
+    double t[kT0Chunk];
+    double target;
+    int first, n;
+};
+hipError_t launch_t0_track(const FieldDesc* fds, const double* out, const T0Args& a, double* rec, hipStream_t stream);
+hipError_t launch_t0_reset(double* rec, int nfields, hipStream_t stream);
 hipError_t launch_diag(int prec, const FieldDesc* fds, const int2* tiles, int ntiles, int ping, double* partials,
                        int nfields, double* out, hipStream_t stream);
 hipError_t launch_to_soa(int prec, const double* src, void* dst, int64_t stride, int s0, int n, hipStream_t stream);

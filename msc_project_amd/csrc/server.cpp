@@ -82,6 +82,7 @@ struct lgf_server {
     uint64_t next_id = 1;
     bool stop = false;
     bool tuning = false;  // lgf_server_autotune is using ctx: admit nothing
+    int fail_batches = 0; // lgf_server_inject_batch_failures: batch creations still to fail
     std::string err;
     std::chrono::steady_clock::time_point t0;
     // counters (guarded by mu)
@@ -178,6 +179,15 @@ static void measure_due(lgf_server* S, const std::vector<JobP>& stepping, const 
 
 static double remaining_work(const Job& j) { return 4.0 * (double)volume_of(j.req.shape) * (j.req.nsteps - j.done); }
 
+// measurements a job records: t = 0, every multiple of measure_every up to
+// nsteps, and the end when it is not such a multiple (measure_every = 0: t = 0
+// and the end)
+static int64_t measurement_count(int nsteps, int every) {
+    if (nsteps == 0) return 1;
+    if (every == 0) return 2;
+    return 1 + nsteps / every + (nsteps % every ? 1 : 0);
+}
+
 static bool due(const Job& j) {
     const int every = j.req.measure_every;
     return j.done == j.req.nsteps || (every > 0 && j.done % every == 0);
@@ -271,7 +281,25 @@ void lgf_server::run() {
                 std::vector<lgf_field*> fs;
                 for (auto& j : stepping)
                     if (j->req.precision == p) fs.push_back(j->field);
-                if (!fs.empty()) lgf_batch_create(ctx, fs.data(), (int)fs.size(), &batch[p]);
+                if (fs.empty()) continue;
+                int rc;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    rc = fail_batches > 0 ? LGF_E_NOMEM : LGF_OK;
+                    if (fail_batches > 0) fail_batches--;
+                }
+                if (rc == LGF_OK) {
+                    rc = lgf_batch_create(ctx, fs.data(), (int)fs.size(), &batch[p]);
+                    if (rc != LGF_OK) set_err(std::string("batch creation failed: ") + lgf_ctx_last_error(ctx));
+                } else {
+                    set_err("batch creation failed (injected fault)");
+                }
+                if (rc != LGF_OK) {
+                    // these jobs cannot step: they fail with the status and are retired below
+                    batch[p] = nullptr;
+                    for (auto& j : stepping)
+                        if (j->req.precision == p) j->status = rc;
+                }
             }
             dirty = false;
         }
@@ -303,9 +331,10 @@ void lgf_server::run() {
         gstep += k;
         std::vector<JobP> done_now, due_now;
         for (auto& j : stepping) {
+            if (j->status != LGF_OK) continue;  // failed: did not step, retired below
             for (int s = 0; s < k; ++s) j->t += j->req.eps;  // the reference's t += eps per step
             j->done += k;
-            if (j->status == LGF_OK && due(*j)) due_now.push_back(j);
+            if (due(*j)) due_now.push_back(j);
         }
         measure_due(this, stepping, due_now, batch);
         for (auto& j : stepping)
@@ -402,6 +431,12 @@ extern "C" int lgf_server_submit(lgf_server* s, const lgf_request* req, uint64_t
     int rc = lgf_field_footprint(req->shape, req->precision, &bytes);
     if (rc) return rc;
     if (req->nsteps < 0 || req->measure_every < 0 || !(req->eps == req->eps)) return LGF_E_ARG;
+    if (measurement_count(req->nsteps, req->measure_every) > kMaxMeasurements) {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->err = "request records " + std::to_string(measurement_count(req->nsteps, req->measure_every)) +
+                 " measurements, more than the " + std::to_string(kMaxMeasurements) + " a result holds";
+        return LGF_E_ARG;
+    }
     const bool fp64_lines = req->precision == LGF_FP64 || (req->measure_flags & LGF_MEASURE_FP64);
     // the line products of one measurement: whole 64-site tiles, as the
     // measurement allocates them
@@ -496,6 +531,13 @@ extern "C" int lgf_server_autotune(lgf_server* s, const int* shapes, int nshapes
     }
     s->cv_work.notify_all();
     return rc;
+}
+
+extern "C" int lgf_server_inject_batch_failures(lgf_server* s, int count) {
+    if (!s || count < 0) return LGF_E_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->fail_batches = count;
+    return LGF_OK;
 }
 
 extern "C" int lgf_server_now(lgf_server* s, double* now_s) {

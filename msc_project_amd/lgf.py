@@ -91,6 +91,10 @@ SIGNATURES = {
     "lgf_field_upload": ([_vp, _dp], ctypes.c_int),
     "lgf_field_download": ([_vp, _dp], ctypes.c_int),
     "lgf_field_set_cold": ([_vp], ctypes.c_int),
+    "lgf_field_flow_time": ([_vp, _dp], ctypes.c_int),
+    "lgf_field_set_flow_time": ([_vp, ctypes.c_double], ctypes.c_int),
+    "lgf_batch_track_t0": ([_vp, ctypes.c_double], ctypes.c_int),
+    "lgf_batch_t0": ([_vp, _dp], ctypes.c_int),
     "lgf_field_set_random": ([_vp, ctypes.c_uint64], ctypes.c_int),
     "lgf_field_heatbath": ([_vp, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int],
                            ctypes.c_int),
@@ -109,6 +113,7 @@ SIGNATURES = {
     "lgf_server_submit": ([_vp, ctypes.POINTER(Request), ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
     "lgf_server_wait": ([_vp, ctypes.c_uint64, ctypes.c_double, ctypes.POINTER(Result)], ctypes.c_int),
     "lgf_server_now": ([_vp, _dp], ctypes.c_int),
+    "lgf_server_inject_batch_failures": ([_vp, ctypes.c_int], ctypes.c_int),
     "lgf_flow_scale_t0": ([_dp, _dp, ctypes.c_int, ctypes.c_double, _dp], ctypes.c_int),
     "lgf_server_set_policy": ([_vp, ctypes.c_int, ctypes.c_int64, ctypes.c_double], ctypes.c_int),
     "lgf_server_autotune": ([_vp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip], ctypes.c_int),
@@ -284,6 +289,17 @@ class Field:
     def set_cold(self):
         self.ctx.check(lib().lgf_field_set_cold(self.h), "set_cold")
 
+    @property
+    def flow_time(self):
+        """t += eps per RK3 step taken (GradientFlow's flow time)"""
+        t = ctypes.c_double()
+        self.ctx.check(lib().lgf_field_flow_time(self.h, ctypes.byref(t)), "flow_time")
+        return t.value
+
+    @flow_time.setter
+    def flow_time(self, t):
+        self.ctx.check(lib().lgf_field_set_flow_time(self.h, float(t)), "set_flow_time")
+
     def set_random(self, seed):
         self.ctx.check(lib().lgf_field_set_random(self.h, int(seed)), "set_random")
 
@@ -402,6 +418,16 @@ class Batch:
         enqueued behind it before the result is collected with measure_finish"""
         self.ctx.check(lib().lgf_batch_measure_start(self.h, flags), "batch measure_start")
 
+    def track_t0(self, target=0.3):
+        """track t0 (t^2 E = target) on the device at every following measurement"""
+        self.ctx.check(lib().lgf_batch_track_t0(self.h, float(target)), "batch track_t0")
+
+    def t0(self):
+        """t0 per field from the on-device tracker (NaN where not reached)"""
+        out = np.empty(len(self.fields))
+        self.ctx.check(lib().lgf_batch_t0(self.h, _p(out)), "batch t0")
+        return out
+
     def measure_finish(self):
         """wait for the outstanding measurement: array (fields x len(OBSERVABLES))"""
         obs = np.empty((len(self.fields), len(OBSERVABLES)))
@@ -475,6 +501,10 @@ class Server:
         return {"id": res.id, "status": res.status, "t": np.array(res.t[:n]), "E": np.array(res.E[:n]),
                 "Q": np.array(res.Q[:n]), "submit_s": res.submit_s, "start_s": res.start_s,
                 "finish_s": res.finish_s}
+
+    def inject_batch_failures(self, count):
+        """test hook: the scheduler's next `count` batch creations fail (LGF_E_NOMEM)"""
+        self._check(lib().lgf_server_inject_batch_failures(self.h, int(count)), "inject_batch_failures")
 
     def now(self):
         t = ctypes.c_double()

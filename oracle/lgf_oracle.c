@@ -38,6 +38,7 @@
  *                       Its only pin is physics: <P> at beta=6.0.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -728,4 +729,9 @@ void orc_generate(const int *shape, double *U, double beta, uint64_t seed, int s
         for (int o = 0; o < or_per_hb; ++o) sweep_all(&L, U, beta, 0, seed, it);
         for (long l = 0; l < L.vol * 4; ++l) orc_reunitarize(U + l * MAT);
     }
+}
+
+/* threads of the OpenMP loops above (timing runs choose their own count) */
+void orc_set_threads(int n) {
+    if (n > 0) omp_set_num_threads(n);
 }

@@ -51,8 +51,14 @@ def lib():
         L.orc_generate.argtypes = [_ip, _dp, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, ctypes.c_int,
                                    ctypes.c_int, ctypes.c_int]
         L.orc_reunitarize.argtypes = [_dp]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
         _lib = L
     return _lib
+
+
+def set_threads(n):
+    """OpenMP threads of the oracle's site loops"""
+    lib().orc_set_threads(int(n))
 
 
 def _shape(shape):

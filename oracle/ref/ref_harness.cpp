@@ -8,12 +8,20 @@
 //   lgf_ref qdensity  <ParseCLargs args...>   -> MeasureQdensityDistribution    (LGF/LGF.cpp:114-124)
 //   lgf_ref resume T  <ParseCLargs args...>   -> ResumeFlowedConfiguration      (LGF/LGF.cpp:197-226)
 //   lgf_ref measured  <ParseCLargs args...>   -> MeasureFlowedGaugeConfigurations (LGF/LGF.cpp:228-255)
+//   lgf_ref timeflow  <ParseCLargs args...>   -> the objects FlowSavedGaugeConfigurations builds for
+//       configuration ConfigStart, then timed: flowSteps calls of GradientFlow::flow() with no
+//       measurement due, and one TopologicalCharge + EnergyDensity calculate() (the measurement the
+//       flow makes every interval); file I/O outside the timed regions.  Times are the slowest
+//       rank's (barriers around each region), printed on stderr.
 //
 // The only behavioural change is std::cout.precision(17), so that the
 // "Topological Charge = " / "Energy Density = " lines printed by the reference
 // (LGF/Observables/TopologicalCharge.cpp:25, EnergyDensity.cpp:30) carry all
 // significant digits.  Wall time of the whole call is printed on stderr.
 #include "LGF.h"
+#include "Gradient_Flow/GradientFlow.h"
+#include "Observables/EnergyDensity.h"
+#include "Observables/TopologicalCharge.h"
 #include <chrono>
 #include <cstring>
 #include <string>
@@ -61,6 +69,26 @@ int main(int argc, char** argv) {
         MeasureFlowedGaugeConfigurations(params, flowParams);
     } else if (mode == "resume") {
         ResumeFlowedConfiguration(params, flowParams, t_pickup);
+    } else if (mode == "timeflow") {
+        Lattice lattice(params, true);
+        Wilson action(params.getbeta());
+        SU3_field U(lattice, params.getextdofs());
+        U.loadSU3FromFile(params.getbeta(), "LHMC_3_of_0_330000", dataFolder, std::to_string(ConfigStart));
+        GradientFlow flowing(action, U, epsilon, INT_MAX);
+        TopologicalCharge topCharge(U);
+        EnergyDensity Edensity(U);
+        MPI_Barrier(mpiWrapper::comm());
+        auto a = std::chrono::steady_clock::now();
+        for (int i = 0; i < flowSteps; i++) flowing.flow();
+        MPI_Barrier(mpiWrapper::comm());
+        auto b = std::chrono::steady_clock::now();
+        topCharge.calculate(flowing.GetFlowTime());
+        Edensity.calculate(flowing.GetFlowTime());
+        MPI_Barrier(mpiWrapper::comm());
+        auto c = std::chrono::steady_clock::now();
+        if (mpiWrapper::id() == 0)
+            std::cerr << "lgf_ref flow_s " << std::chrono::duration<double>(b - a).count() << " steps " << flowSteps
+                      << " measure_s " << std::chrono::duration<double>(c - b).count() << "\n";
     } else {
         if (mpiWrapper::id() == 0) std::cerr << "unknown mode " << mode << "\n";
         mpiWrapper::end_parallelSession();

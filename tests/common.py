@@ -50,6 +50,34 @@ def q_err(Q, Qref, Qabs):
     return np.max(np.abs(Q - Qref) / np.maximum(np.abs(Qref), Qabs))
 
 
+def q_rel_strict(Q, Qref, Qabs, floor=1e-3):
+    """max |dQ|/|Q| over the measurements where |Q| is at least `floor` times
+    sum_x |q(x)|/(4 pi^2), and how many there are.  Below that floor Q is a
+    near-cancelling sum, and the rounding of the reference's own sum
+    (~1e-16 * sum |q|) exceeds 1e-12 |Q|."""
+    Q, Qref, Qabs = (np.asarray(x, dtype=np.float64) for x in (Q, Qref, Qabs))
+    sel = np.abs(Qref) >= floor * Qabs
+    if not sel.any():
+        return 0.0, 0
+    return float(np.max(np.abs(Q[sel] - Qref[sel]) / np.abs(Qref[sel]))), int(sel.sum())
+
+
+def flow_measure(field, eps, steps, every, flags=0):
+    """E, Q and the |q| scale at t = 0 and every `every` RK3 steps of a GPU
+    field (msc_project_amd.lgf.Field); flags: lgf measure flags"""
+    t, E, Q, Qa = [0.0], [], [], []
+    e, q, qa, _ = field.measure(flags)
+    E.append(e), Q.append(q), Qa.append(qa)
+    tt = 0.0
+    for s in range(1, steps + 1):
+        field.flow(eps, 1)
+        tt += eps
+        if s % every == 0:
+            e, q, qa, _ = field.measure(flags)
+            t.append(tt), E.append(e), Q.append(q), Qa.append(qa)
+    return np.array(t), np.array(E), np.array(Q), np.array(Qa)
+
+
 def haar_field(shape, seed):
     from oracle import oracle as O
     return O.generate(shape, 6.0, seed=seed, start="hot", sweeps=0)

@@ -3,7 +3,7 @@
 Run in the build container only (it needs oracle/_ref/lgf_ref, which
 `make -C oracle/ref` builds from the reference sources):
 
-    python tests/golden/make_golden.py [small|config1|large24|all]
+    python tests/golden/make_golden.py [small|config1|large24|resume|all]
 
 Inputs are produced by the seeded generator of the CPU restatement
 (oracle/lgf_oracle.c, orc_generate); their sha256 is recorded so that a test
@@ -176,11 +176,55 @@ def case_large24(steps=50):
     return res
 
 
+def case_resume():
+    """ResumeFlowedConfiguration (LGF/LGF.cpp:197-226) on the 4^4 fixture: the
+    reference flows it 10 RK3 steps (eps 0.02, every 5), then resumes the
+    saved configuration at t = 0.2 for 10 more steps.  Recorded: the Q and E
+    the resume prints (17 digits; it does not measure at the pickup time), the
+    CSVs after both runs (the resume appends to them), and the configuration
+    the resume saves."""
+    shape, beta, eps, every = (4, 4, 4, 4), 6.0, 0.02, 5
+    U = np.fromfile(os.path.join(HERE, "cfg_4x4x4x4_hot5.bin"), dtype=np.float64)
+    folder = tempfile.mkdtemp(prefix="lgf_golden_") + "/"
+    try:
+        ens = os.path.join(folder, "ensembles", "su3", "beta" + beta_str(beta), shape_str(shape), IN_METHOD)
+        os.makedirs(ens)
+        U.tofile(os.path.join(ens, "torus_extdof4_0.bin"))
+        run_ref("flow", shape, beta, 0, 0, 10, eps, every, folder)
+        tt = 0.0
+        for _ in range(10):
+            tt += eps
+        pickup = "%f" % tt  # std::to_string of the flow time, the saved file's name
+        Q, E, wall, _ = run_ref("resume", shape, beta, 0, 0, 10, eps, every, folder, extra=(pickup,))
+        t = []
+        for s in range(1, 11):
+            tt += eps
+            if s % every == 0:
+                t.append(tt)
+        obs = os.path.join(folder, "Observables")
+        gf = os.path.join(folder, "ensembles", "su3", "beta" + beta_str(beta), shape_str(shape), "GF")
+        flowed = os.path.join(gf, "torus_extdof4_0_Flowtime%f.bin" % tt)
+        fl = np.fromfile(flowed, dtype=np.float64)
+        res = {"name": "resume_4x4x4x4", "shape": list(shape), "beta": beta, "eps": eps, "every": every,
+               "first_steps": 10, "resume_steps": 10, "pickup": pickup, "input_file": "cfg_4x4x4x4_hot5.bin",
+               "t": t, "Q": Q, "E": E,
+               "csv_Q": open(os.path.join(obs, "Topological_Charge", "beta" + beta_str(beta), shape_str(shape),
+                                          "GF", "torus_extdof4_0.csv")).read(),
+               "csv_E": open(os.path.join(obs, "EnergyDensity", "beta" + beta_str(beta), shape_str(shape),
+                                          "GF", "torus_extdof4_0.csv")).read(),
+               "flowed_name": os.path.basename(flowed), "flowed_sha256": sha(flowed),
+               "flowed_first_link": fl[:18].tolist(), "flowed_last_link": fl[-18:].tolist(),
+               "ref_wall_s_1rank": wall}
+        return res
+    finally:
+        shutil.rmtree(folder, ignore_errors=True)
+
+
 def main():
     which = sys.argv[1] if len(sys.argv) > 1 else "all"
     if not os.path.exists(REF):
         raise SystemExit("build the reference first: make -C oracle/ref")
-    cases = {"small": case_small, "config1": case_config1, "large24": case_large24}
+    cases = {"small": case_small, "config1": case_config1, "large24": case_large24, "resume": case_resume}
     for k, fn in cases.items():
         if which in (k, "all"):
             t0 = time.time()

@@ -6,6 +6,8 @@ import os
 import subprocess
 import sys
 
+import numpy as np
+
 from msc_project_amd import lgf
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -46,3 +48,31 @@ def test_more_than_one_gpu_is_refused():
     r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_accuracy_reports_t0_when_the_series_reaches_it():
+    """The FP32-vs-FP64 comparison carries t0 (t^2 E = 0.3, linear
+    interpolation) and |dQ|, rel |d(t^2 E)| interpolated there."""
+    b = load_bench()
+    t = np.arange(0, 5.0001, 0.05)
+    E64 = 0.3 / 3.2 ** 2 * (3.2 / np.maximum(t, 1e-9)) ** 1.5  # t^2 E = 0.3 at t = 3.2
+    E64[0] = 1.0
+    r64 = {"t": t, "E": E64, "Q": np.full_like(t, 1.5)}
+    r32 = {"t": t, "E": E64 * (1 + 1e-8), "Q": np.full_like(t, 1.5 + 2e-6)}
+    a = b.accuracy(r64, r32)
+    assert abs(a["t0"] - 3.2) < 1e-3
+    assert abs(a["abs_dQ_at_t0"] - 2e-6) < 1e-12
+    assert abs(a["rel_dt2E_at_t0"] - 1e-8) < 1e-12
+    # a series that stops before t0 carries none
+    a = b.accuracy({k: v[:20] for k, v in r64.items()}, {k: v[:20] for k, v in r32.items()})
+    assert "t0" not in a
+
+
+def test_reference_record_names_host_ranks_and_cores():
+    b = load_bench()
+    rec = b.reference_record()
+    assert rec is not None, "profiles/cpu_reference_flow_24x24x24x24.json missing"
+    assert rec["kind"] == "reference" and "cpu" in rec["host"]
+    for ranks in ("1", "8"):
+        r = rec["ranks_%s" % ranks]
+        assert r["cores"] == int(ranks) and 0 < r["job_measure_every_5"] < r["flow_only"]

@@ -41,14 +41,33 @@ def read_full(path):
     return np.array([float(r[0]) for r in rows]), np.array([float(r[1]) for r in rows])
 
 
-def csv_equal_or_rounding_tie(text, ref_text, full_vals):
-    if text == ref_text:
-        return True
-    # 6-significant-digit text may differ only where the value sits at a rounding tie
-    for a, b, v in zip(text.split(), ref_text.split(), full_vals):
-        if a != b and abs(float(a.split(",")[1]) - float(b.split(",")[1])) > 1.01e-6 * max(abs(v), 1e-300) * 10:
+def csv_matches_reference(text, ref_text, ref_full, tol=TOL_FP64):
+    """Our 6-significant-digit CSV against the reference's, line by line.
+    Lines must be identical, except at a genuine rounding tie: a value and
+    one within `tol` relative of it print differently only where a rounding
+    boundary of the 6-digit print lies between them, so a differing line is
+    accepted only if the two prints are exactly the two 6-digit values of the
+    window [v - tol |v|, v + tol |v|] around the reference's full-precision
+    value v (the reference prints with the default 6 significant digits,
+    %g-style, LGF/Observables/TopologicalCharge.cpp:31-71)."""
+    a, b = text.split(), ref_text.split()
+    if len(a) != len(b) or len(b) != len(ref_full):
+        return False
+    for la, lb, v in zip(a, b, ref_full):
+        if la == lb:
+            continue
+        ta, xa = la.split(",")
+        tb, xb = lb.split(",")
+        window = {"%.6g" % (v - tol * abs(v)), "%.6g" % (v + tol * abs(v))}
+        if ta != tb or len(window) != 2 or {xa, xb} != window:
             return False
     return True
+
+
+def q_close(Q, Qref, tol=TOL_FP64, floor=1e-3):
+    """|dQ| <= tol * max(|Q|, floor): relative wherever |Q| >= floor"""
+    Q, Qref = np.asarray(Q, dtype=np.float64), np.asarray(Qref, dtype=np.float64)
+    return bool(np.all(np.abs(Q - Qref) <= tol * np.maximum(np.abs(Qref), floor)))
 
 
 @pytest.mark.parametrize("mode", ["flow", "batch"])
@@ -69,9 +88,10 @@ def test_host_flow_config1(tmp_path, mode):
         tq, Q = read_full(os.path.join(qdir, "torus_extdof4_%d.full.csv" % i))
         te, E = read_full(os.path.join(edir, "torus_extdof4_%d.full.csv" % i))
         assert rel_err(E, c["E"]) < TOL_FP64
-        assert np.max(np.abs(Q - np.asarray(c["Q"]))) < TOL_FP64 * max(1.0, np.max(np.abs(c["Q"])))
-        assert csv_equal_or_rounding_tie(open(os.path.join(qdir, "torus_extdof4_%d.csv" % i)).read(), c["csv_Q"], Q)
-        assert csv_equal_or_rounding_tie(open(os.path.join(edir, "torus_extdof4_%d.csv" % i)).read(), c["csv_E"], E)
+        assert q_close(Q, c["Q"])
+        assert np.allclose(tq, c["t"], rtol=0, atol=1e-12) and np.allclose(te, c["t"], rtol=0, atol=1e-12)
+        assert csv_matches_reference(open(os.path.join(qdir, "torus_extdof4_%d.csv" % i)).read(), c["csv_Q"], c["Q"])
+        assert csv_matches_reference(open(os.path.join(edir, "torus_extdof4_%d.csv" % i)).read(), c["csv_E"], c["E"])
         flowed = os.path.join(folder, "ensembles", "su3", beta_dir(d["beta"]), sd, "GF", c["flowed_name"])
         assert os.path.getsize(flowed) == np.prod(shape) * 576
 
@@ -142,9 +162,11 @@ def test_host_measured_small(tmp_path):
     _, Q = read_full(os.path.join(qdir, "torus_extdof4_0.full.csv"))
     _, E = read_full(os.path.join(edir, "torus_extdof4_0.full.csv"))
     assert rel_err(E, [c["measured_E_improved"]]) < TOL_FP64
-    assert abs(Q[0] - c["measured_Q_unimproved"]) < TOL_FP64 * max(1.0, abs(c["measured_Q_unimproved"]))
-    assert csv_equal_or_rounding_tie(open(os.path.join(qdir, "torus_extdof4_0.csv")).read(), c["csv_Q_unimproved"], Q)
-    assert csv_equal_or_rounding_tie(open(os.path.join(edir, "torus_extdof4_0.csv")).read(), c["csv_E_improved"], E)
+    assert q_close(Q, [c["measured_Q_unimproved"]])
+    assert csv_matches_reference(open(os.path.join(qdir, "torus_extdof4_0.csv")).read(), c["csv_Q_unimproved"],
+                                 [c["measured_Q_unimproved"]])
+    assert csv_matches_reference(open(os.path.join(edir, "torus_extdof4_0.csv")).read(), c["csv_E_improved"],
+                                 [c["measured_E_improved"]])
 
 
 def test_host_resume_continues_flow(tmp_path):
@@ -203,3 +225,44 @@ def test_host_missing_input_errors(tmp_path):
     out = subprocess.run([LGF, "4", "4", "6.0", "4", "4", "4", "4", "0", "0", "0", "0", "0", "0", "5", "0.02", "5",
                           str(tmp_path) + "/"], capture_output=True, text=True, timeout=120)
     assert out.returncode != 0 and "cannot open" in out.stderr
+
+
+def test_host_resume_matches_reference(tmp_path):
+    """ResumeFlowedConfiguration against the reference's own resume
+    (tests/golden/ref_resume.json, LGF/LGF.cpp:197-226 with
+    GradientFlow::SetFlowTime, LGF/Gradient_Flow/GradientFlow.cpp:64-66):
+    10 RK3 steps, then a resume from the configuration saved at t = 0.2 for
+    10 more.  The resume does not measure at the pickup time and appends to
+    the same CSVs; the 6-digit CSVs must match the reference's, the resumed
+    full-precision rows its 17-digit output within the FP64 bar, and the
+    configuration the resume saves (its name carries the new flow time) the
+    reference's at its first and last link."""
+    d = golden("resume")
+    s_d, U = small_input()  # the same 4^4 input; its golden's first 10 steps are the first run
+    shape = tuple(d["shape"])
+    folder = str(tmp_path) + "/"
+    place_inputs(folder, shape, d["beta"], [U])
+    run_lgf(folder, shape, d["beta"], 0, d["first_steps"], d["eps"], d["every"])
+    out = run_lgf(folder, shape, d["beta"], 0, d["resume_steps"], d["eps"], d["every"], "--mode=resume",
+                  "--pickup=" + d["pickup"])
+    lines = [l for l in out.splitlines() if l.startswith("Topological Charge") or l.startswith("Energy Density")]
+    assert len(lines) == 2 * len(d["t"])  # no measurement at the pickup time
+    sd = "X".join(map(str, shape))
+    obs = os.path.join(folder, "Observables")
+    qdir = os.path.join(obs, "Topological_Charge", beta_dir(d["beta"]), sd, "GF")
+    edir = os.path.join(obs, "EnergyDensity", beta_dir(d["beta"]), sd, "GF")
+    n0 = 1 + d["first_steps"] // d["every"]
+    sc = s_d["configs"][0]
+    Qref = list(sc["Q"][:n0]) + list(d["Q"])
+    Eref = list(sc["E"][:n0]) + list(d["E"])
+    tq, Q = read_full(os.path.join(qdir, "torus_extdof4_0.full.csv"))
+    te, E = read_full(os.path.join(edir, "torus_extdof4_0.full.csv"))
+    assert np.allclose(tq[n0:], d["t"], rtol=0, atol=1e-12) and np.allclose(te[n0:], d["t"], rtol=0, atol=1e-12)
+    assert rel_err(E, Eref) < TOL_FP64
+    assert q_close(Q, Qref)
+    assert csv_matches_reference(open(os.path.join(qdir, "torus_extdof4_0.csv")).read(), d["csv_Q"], Qref)
+    assert csv_matches_reference(open(os.path.join(edir, "torus_extdof4_0.csv")).read(), d["csv_E"], Eref)
+    fl = np.fromfile(os.path.join(folder, "ensembles", "su3", beta_dir(d["beta"]), sd, "GF", d["flowed_name"]))
+    assert fl.size == np.prod(shape) * 72
+    assert np.max(np.abs(fl[:18] - d["flowed_first_link"])) < 1e-13
+    assert np.max(np.abs(fl[-18:] - d["flowed_last_link"])) < 1e-13

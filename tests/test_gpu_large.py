@@ -10,7 +10,9 @@ sites.  The reference itself cannot load or save a 64^4 configuration
 (32-bit file offsets, LGF/Field/SU3_field.cpp:156,331), so this case has no
 reference golden vector.
 
-Slow (minutes, ~60 GB of device memory): runs with LGF_SLOW=1.
+About 100 s of GPU time and ~60 GB of device memory in all, split into one
+test per flow time (sharing the lattices through a module fixture) so that
+each reports as it ends.
 """
 import os
 
@@ -40,10 +42,8 @@ def site_q(shape, U, g):
     return s
 
 
-@pytest.mark.slow
-def test_config5_qdensity_64(tmp_path):
-    if not os.environ.get("LGF_SLOW"):
-        pytest.skip("set LGF_SLOW=1: 64^4 lattice, minutes of GPU time")
+@pytest.fixture(scope="module")
+def lattices():
     ctx = lgf.Context(0)
     ctx.autotune(SHAPE, lgf.FP32, 2)
     f64 = lgf.Field(ctx, SHAPE, lgf.FP64)
@@ -52,48 +52,60 @@ def test_config5_qdensity_64(tmp_path):
     ctx.synchronize()
     plaq = f64.diagnostics()[0]
     print("64^4 input: beta %.2f, 100 HB/OR iterations, <P> = %.6f" % (BETA, plaq), flush=True)
-    assert 0.60 < plaq < 0.63
     f32 = lgf.Field(ctx, SHAPE, lgf.FP32)
     f32.copy_from(f64)
-    scale = np.pi ** 2 / 4  # q(x) file scale of the reference
-    steps_done = 0
-    for i, t in enumerate(TIMES):
-        n = int(round(t / EPS)) - steps_done
-        f64.flow(EPS, n)
-        f32.flow(EPS, n)
-        steps_done += n
-        o64 = f64.observables(lgf.MEASURE_DEFAULT, want_q=True)
-        o32 = f32.observables(lgf.MEASURE_FP64, want_q=True)
-        # q(x) files in the reference's format, read back
-        paths = []
-        for name, o in (("fp64", o64), ("fp32", o32)):
-            p = str(tmp_path / ("torus_extdof1_0_Flowtime%f_%s.bin" % (t, name)))
-            (o["qraw"] * scale).astype(np.float64).tofile(p)
-            paths.append(p)
-        q64 = np.fromfile(paths[0], dtype=np.float64)
-        q32 = np.fromfile(paths[1], dtype=np.float64)
-        assert q64.size == np.prod(SHAPE) and q32.size == np.prod(SHAPE)
-        for p in paths:
-            os.remove(p)
-        Qabs = o64["Qabs"]
-        # the file's sum is pi^4 Q (pi^2/4 scale times 4 pi^2 Q)
-        assert abs(q64.sum() / (scale * 4 * np.pi ** 2) - o64["Q"]) < 1e-10 * Qabs
-        dQ = abs(o32["Q"] - o64["Q"])
-        dt2E = abs(o32["E"] - o64["E"]) / abs(o64["E"])
-        dq = np.max(np.abs(q32 - q64)) / np.max(np.abs(q64))
-        print("t = %.0f: Q64 %.6f Q32 %.6f |dQ| %.2e  t2E %.6f rel d(t2E) %.2e  max|dq|/max|q| %.2e" %
-              (t, o64["Q"], o32["Q"], dQ, t * t * o64["E"], dt2E, dq), flush=True)
-        assert dQ < 1e-4
-        assert dt2E < 1e-6
-        assert dq < 1e-3
-        if i == 0:
-            U = f64.download()
-            V = np.prod(SHAPE)
-            for g in (0, 12345, V // 2 + 777, V - 1):
-                qo = site_q(SHAPE, U, g)
-                assert abs(o64["qraw"][g] - qo) < 1e-12 * np.max(np.abs(o64["qraw"])), g
-            del U
-            print("t = 1: FP64 q(x) equals the oracle's at 4 sites", flush=True)
+    state = {"ctx": ctx, "f64": f64, "f32": f32, "steps": 0, "plaq": plaq}
+    yield state
     f32.close()
     f64.close()
     ctx.close()
+
+
+def test_config5_input_thermalised(lattices):
+    # <P> of beta 6.2 after 100 iterations from a cold start (literature: 0.6136 at equilibrium)
+    assert 0.60 < lattices["plaq"] < 0.63
+
+
+@pytest.mark.parametrize("t", TIMES)
+def test_config5_qdensity_64(lattices, t, tmp_path):
+    f64, f32 = lattices["f64"], lattices["f32"]
+    n = int(round(t / EPS)) - lattices["steps"]
+    assert n > 0
+    f64.flow(EPS, n)
+    f32.flow(EPS, n)
+    lattices["steps"] += n
+    o64 = f64.observables(lgf.MEASURE_DEFAULT, want_q=True)
+    o32 = f32.observables(lgf.MEASURE_FP64, want_q=True)
+    scale = np.pi ** 2 / 4  # q(x) file scale of the reference
+    # q(x) files in the reference's format, read back
+    paths = []
+    for name, o in (("fp64", o64), ("fp32", o32)):
+        p = str(tmp_path / ("torus_extdof1_0_Flowtime%f_%s.bin" % (t, name)))
+        (o["qraw"] * scale).astype(np.float64).tofile(p)
+        paths.append(p)
+    q64 = np.fromfile(paths[0], dtype=np.float64)
+    q32 = np.fromfile(paths[1], dtype=np.float64)
+    assert q64.size == np.prod(SHAPE) and q32.size == np.prod(SHAPE)
+    for p in paths:
+        os.remove(p)
+    Qabs = o64["Qabs"]
+    # the file's sum is pi^4 Q (pi^2/4 scale times 4 pi^2 Q)
+    assert abs(q64.sum() / (scale * 4 * np.pi ** 2) - o64["Q"]) < 1e-10 * Qabs
+    dQ = abs(o32["Q"] - o64["Q"])
+    dt2E = abs(o32["E"] - o64["E"]) / abs(o64["E"])
+    dq = np.max(np.abs(q32 - q64)) / np.max(np.abs(q64))
+    print("t = %.0f: Q64 %.6f Q32 %.6f |dQ| %.2e  t2E %.6f rel d(t2E) %.2e  max|dq|/max|q| %.2e" %
+          (t, o64["Q"], o32["Q"], dQ, t * t * o64["E"], dt2E, dq), flush=True)
+    assert dQ < 1e-4
+    assert dt2E < 1e-6
+    assert dq < 1e-3
+    if t == TIMES[0]:
+        # the FP64 path's per-site q(x) against the oracle, at sites spread over
+        # the whole index range (the last one exercises the largest offsets)
+        U = f64.download()
+        V = np.prod(SHAPE)
+        for g in (0, 12345, V // 2 + 777, V - 1):
+            qo = site_q(SHAPE, U, g)
+            assert abs(o64["qraw"][g] - qo) < 1e-12 * np.max(np.abs(o64["qraw"])), g
+        del U
+        print("t = 1: FP64 q(x) equals the oracle's at 4 sites", flush=True)

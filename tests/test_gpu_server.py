@@ -100,3 +100,91 @@ def test_server_autotune_only_when_idle():
     t, E, Q = alone(ctx, shape, eps, n, every, prec, seed=seed)
     ctx.close()
     assert np.array_equal(r["t"], t) and np.array_equal(r["E"], E) and np.array_equal(r["Q"], Q)
+
+
+def test_server_rejects_requests_beyond_result_capacity():
+    """A result holds 1024 measurements; a request needing more is refused
+    at submit (LGF_E_ARG), one needing exactly 1024 is accepted."""
+    srv = lgf.Server(0)
+    with pytest.raises(lgf.LgfError) as e:
+        srv.submit((4, 4, 4, 4), 0.01, 2000, 1, seed=1)  # 2001 measurements
+    assert e.value.code == 1
+    with pytest.raises(lgf.LgfError) as e:
+        srv.submit((4, 4, 4, 4), 0.01, 1024, 1, seed=1)  # t = 0 plus 1024
+    assert e.value.code == 1
+    i = srv.submit((4, 4, 4, 4), 0.001, 1023, 1, seed=1)  # exactly 1024
+    r = srv.wait(i, timeout=120)
+    srv.close()
+    assert r["status"] == 0 and len(r["t"]) == 1024
+
+
+def test_server_batch_creation_failure_fails_its_jobs():
+    """When the scheduler cannot create the batch a job steps in, that job
+    ends with the batch's status (LGF_E_NOMEM) after its t = 0 measurement,
+    without stepping, its memory is freed, and later jobs run normally."""
+    srv = lgf.Server(0)
+    srv.inject_batch_failures(1)
+    a = srv.submit((8, 8, 8, 8), 0.01, 20, 10, precision=lgf.FP32, seed=5)
+    ra = srv.wait(a, timeout=120)
+    assert ra["status"] == 4  # LGF_E_NOMEM
+    assert np.array_equal(ra["t"], [0.0])
+    b = srv.submit((8, 8, 8, 8), 0.01, 20, 10, precision=lgf.FP32, seed=5)
+    rb = srv.wait(b, timeout=120)
+    st = srv.stats()
+    srv.close()
+    assert rb["status"] == 0 and len(rb["t"]) == 3
+    assert rb["E"][0] == ra["E"][0] and rb["Q"][0] == ra["Q"][0]
+    assert st["jobs_done"] == 2
+    ctx = lgf.Context(0)
+    t, E, Q = alone(ctx, (8, 8, 8, 8), 0.01, 20, 10, lgf.FP32, seed=5)
+    ctx.close()
+    assert np.array_equal(rb["t"], t) and np.array_equal(rb["E"], E) and np.array_equal(rb["Q"], Q)
+
+
+# shapes and couplings of the serving stream (the 500-request Poisson stream
+# of the benchmark draws from these)
+STREAM_SHAPES = [(16, 16, 16, 16), (24, 24, 24, 24), (32, 32, 32, 32), (32, 32, 32, 64)]
+STREAM_BETAS = [5.96, 6.05, 6.13, 6.21]
+
+
+def test_server_stream_shapes_same_alone_and_in_mix():
+    """Jobs of the serving stream's lattice shapes (16^4 .. 32^3x64), short
+    FP32 flows with mixed step sizes arriving staggered while others run:
+    each job's E(t), Q(t) are bitwise those of the same job flowed alone, and
+    its Q tracks the FP64 flow of the same input within |dQ| < 1e-4 and its
+    E within 1e-6 relative."""
+    rng = np.random.default_rng(20261204)
+    ctx = lgf.Context(0)
+    jobs = []
+    for k in range(7):
+        shape = STREAM_SHAPES[k % 4] if k < 4 else STREAM_SHAPES[rng.integers(4)]
+        beta = STREAM_BETAS[rng.integers(4)]
+        eps = (0.01, 0.02)[k % 2]
+        nsteps = int(rng.integers(2, 5)) * 10
+        g = lgf.Field(ctx, shape, lgf.FP64)
+        g.set_cold()
+        g.heatbath(beta, 7000 + k, sweeps=10, or_per_hb=4)
+        jobs.append((shape, beta, eps, nsteps, g.download()))
+        g.close()
+    srv = lgf.Server(0)
+    ids = []
+    for k, (shape, beta, eps, nsteps, U) in enumerate(jobs):
+        ids.append(srv.submit(shape, eps, nsteps, 10, precision=lgf.FP32, links=U, beta=beta))
+        time.sleep(0.02)  # later jobs join a running batch
+    res = [srv.wait(i, timeout=300) for i in ids]
+    st = srv.stats()
+    srv.close()
+    assert st["jobs_done"] == len(jobs)
+    for (shape, beta, eps, nsteps, U), r in zip(jobs, res):
+        assert r["status"] == 0
+        assert len(r["t"]) == 1 + nsteps // 10
+        t, E, Q = alone(ctx, shape, eps, nsteps, 10, lgf.FP32, links=U)
+        assert np.array_equal(r["t"], t)
+        assert np.array_equal(r["E"], E) and np.array_equal(r["Q"], Q), shape
+        t64, E64, Q64 = alone(ctx, shape, eps, nsteps, 10, lgf.FP64, links=U)
+        dQ = np.max(np.abs(r["Q"] - Q64))
+        dE = np.max(np.abs(r["E"] - E64) / np.abs(E64))
+        print("%s eps %.2f %d steps: max |dQ| %.2e, max rel dE %.2e vs FP64" % (shape, eps, nsteps, dQ, dE),
+              flush=True)
+        assert dQ < 1e-4 and dE < 1e-6
+    ctx.close()
