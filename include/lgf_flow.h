@@ -184,6 +184,19 @@ int lgf_measure(lgf_field* f, int flags, double* E, double* Q, double* Qabs, dou
  * (LGF/Observables/UnImprovedTopologicalCharge.cpp:8-29), the observables of
  * MeasureFlowedGaugeConfigurations (LGF/LGF.cpp:228-255) */
 int lgf_measure_observables(lgf_field* f, int flags, double* obs, double* qraw);
+/* Allocation-free lgf_measure_observables (without q(x)).  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Its scratch (descriptor, tile list, clover line products,
+ * tile partials, sums) is a static device array of 256 MiB that each GPU the
+ * library runs on holds until the process ends; it calls no hipMalloc or
+ * hipFree.  Fields up to 465408 sites (FP32 clover arithmetic) or 232832
+ * sites (FP64 arithmetic: LGF_FP64 fields or LGF_MEASURE_FP64) fit; larger
+ * ones give LGF_E_ARG.  The observables are bitwise those of
+ * lgf_measure_observables.  Calls on one GPU share the array and are
+ * serialised within the process; each returns after its results are back
+ * on the host. */
+int lgf_measure_observables_noalloc(lgf_field* f, int flags, double* obs);
 int lgf_diagnostics(lgf_field* f, double* avg_plaquette, double* unitarity_max, double* unitarity_mean);
 /* flow-time scale t0 of a measured series: the t at which t^2 E(t) first
  * reaches `target` (0.3 defines t0, doc/thesis.tex:1214-1216; the reference

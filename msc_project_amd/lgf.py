@@ -79,6 +79,7 @@ SIGNATURES = {
     "lgf_ctx_get_tile_order": ([_vp, _ip], ctypes.c_int),
     "lgf_ctx_set_measure_scratch": ([_vp, ctypes.c_size_t], ctypes.c_int),
     "lgf_measure_observables": ([_vp, ctypes.c_int, _dp, _dp], ctypes.c_int),
+    "lgf_measure_observables_noalloc": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
     "lgf_batch_measure_observables": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
     "lgf_batch_measure_start": ([_vp, ctypes.c_int], ctypes.c_int),
     "lgf_batch_measure_finish": ([_vp, _dp], ctypes.c_int),
@@ -337,6 +338,13 @@ class Field:
         out = dict(zip(OBSERVABLES, obs.tolist()))
         out["qraw"] = q
         return out
+
+    def observables_noalloc(self, flags=MEASURE_DEFAULT):
+        """observables() through the allocation-free entry point (no q(x)):
+        its scratch is the library's static device array"""
+        obs = np.empty(len(OBSERVABLES))
+        self.ctx.check(lib().lgf_measure_observables_noalloc(self.h, flags, _p(obs)), "measure_observables_noalloc")
+        return dict(zip(OBSERVABLES, obs.tolist()))
 
     def measure_start(self, flags=MEASURE_DEFAULT):
         """enqueue the clover sweep of the current links; collect with measure_finish"""
