@@ -257,7 +257,8 @@ def batched(lgf, ctx):
                                   "576 B, the accumulator as 9 reals",
                          "time": "wall clock of 60 graph-replayed RK3 steps of the whole batch"},
             "steps": 300, "measure_every": 10, "precision": "fp32",
-            "tuning": {"stage_variant": best, "tile_order": ctx.tile_order()}}
+            "tuning": {"stage_variant": best, "tile_order": ctx.tile_order(),
+                       "cache_group_bytes": ctx.batch_group_bytes()}}
 
 
 def serving(lgf, nreq, throughput, seed=4242, policy=1, age_rate=0.0):

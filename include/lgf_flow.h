@@ -112,6 +112,16 @@ int lgf_num_stage_variants(int precision, int* n);
  * lattices whose x2-x3 planes are large); results do not depend on it */
 int lgf_ctx_set_tile_order(lgf_ctx* ctx, int block);
 int lgf_ctx_get_tile_order(lgf_ctx* ctx, int* block);
+/* cache groups of batches created afterwards: the fields of a batch are
+ * split, in order, into groups whose device footprint (lgf_field_footprint)
+ * stays within `bytes` (a larger field forms a group of its own), and each
+ * group takes all the steps of an lgf_batch_flow_steps call before the next
+ * group starts, so that a group that fits the GPU's last-level cache
+ * (256 MB) stays there across its stages.  0 = the whole batch is one group
+ * (the default).  Results do not depend on it; lgf_autotune_batch chooses
+ * it. */
+int lgf_ctx_set_batch_group_bytes(lgf_ctx* ctx, size_t bytes);
+int lgf_ctx_get_batch_group_bytes(lgf_ctx* ctx, size_t* bytes);
 /* device scratch a measurement may hold for its line products (2- and
  * 3-link products of every link, 576 B per site for FP32 clover arithmetic,
  * 1152 B for FP64): the fields of a batch are measured in consecutive groups
@@ -128,10 +138,14 @@ int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, in
 /* the same search on `nshapes` seeded lattices (shapes[4*i .. 4*i+3])
  * flowed together as one batch: tunes the context for batches of that mix
  * of shapes, whose best order and variant can differ from any one
- * lattice's.  The lattices of all shapes are held at once (the sum of their
+ * lattice's.  With more than one lattice the pairs are timed with the
+ * batch as one group and in cache groups of 224 MiB
+ * (lgf_ctx_set_batch_group_bytes), and the cap is then chosen among 0, 128,
+ * 192, 224 and 256 MiB at the best pair; ms_per_step is at the best cap of
+ * the search.  The lattices of all shapes are held at once (the sum of their
  * lgf_field_footprint) and released before it returns.  LGF_E_ARG for
  * nshapes <= 0, LGF_E_SHAPE for a bad shape; on failure the context's
- * tile order and variant are left as they were. */
+ * tile order, variant and group cap are left as they were. */
 int lgf_autotune_batch(lgf_ctx* ctx, const int* shapes, int nshapes, int precision, int nsteps, int* best,
                        double* ms_per_step);
 

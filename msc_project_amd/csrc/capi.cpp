@@ -533,6 +533,26 @@ extern "C" int lgf_batch_create(lgf_ctx* ctx, lgf_field* const* fields, int nfie
         for (int t : batch_tile_order(f, ctx->tile_block)) tiles.push_back(make_int2(i, t));
     }
     b->ntiles = (int)tiles.size();
+    // cache groups: a field that alone exceeds the cap forms a group of its own
+    {
+        int first = 0, nt = 0;
+        size_t bytes = 0;
+        for (int i = 0; i < nfields; ++i) {
+            size_t fb = 0;
+            (void)lgf_field_footprint(fields[i]->shape, prec, &fb);
+            if (ctx->group_bytes && nt > 0 && bytes + fb > ctx->group_bytes) {
+                b->groups.push_back(make_int2(first, nt));
+                first += nt;
+                nt = 0;
+                bytes = 0;
+            }
+            nt += fields[i]->ntiles;
+            bytes += fb;
+        }
+        b->groups.push_back(make_int2(first, nt));
+    }
+    b->graph.assign(2 * b->groups.size(), nullptr);
+    b->graph_variant.assign(2 * b->groups.size(), -1);
     b->ping = 0;
     hipError_t e = hipMalloc(&b->d_desc, sizeof(FieldDesc) * nfields);
     if (e == hipSuccess) e = hipMalloc(&b->d_tiles, sizeof(int2) * tiles.size());
@@ -562,8 +582,8 @@ extern "C" int lgf_batch_destroy(lgf_batch* b) {
     }
     (void)hipSetDevice(b->ctx->device);
     (void)hipStreamSynchronize(b->ctx->stream);
-    for (int i = 0; i < 2; ++i)
-        if (b->graph[i]) (void)hipGraphExecDestroy(b->graph[i]);
+    for (hipGraphExec_t g : b->graph)
+        if (g) (void)hipGraphExecDestroy(g);
     if (b->d_desc) (void)hipFree(b->d_desc);
     if (b->d_tiles) (void)hipFree(b->d_tiles);
     if (b->d_partials) (void)hipFree(b->d_partials);
@@ -602,11 +622,12 @@ int batch_sync_desc(lgf_batch* b, const double* eps) {
     return LGF_OK;
 }
 
-static int enqueue_step(lgf_batch* b, int ping) {
+// one RK3 step of the fields of tiles [t0, t0 + nt) of the tile list
+static int enqueue_step(lgf_batch* b, int ping, int t0, int nt) {
     lgf_ctx* ctx = b->ctx;
     const int var = ctx->stage_variant[b->prec];
     for (int stage = 0; stage < 3; ++stage) {
-        hipError_t e = launch_rk3_stage_variant(b->prec, var, stage, b->d_desc, b->d_tiles, b->ntiles, ping ^ (stage & 1),
+        hipError_t e = launch_rk3_stage_variant(b->prec, var, stage, b->d_desc, b->d_tiles + t0, nt, ping ^ (stage & 1),
                                                 ctx->stream);
         if (e != hipSuccess) return lgf_hip_fail(ctx, e, "rk3 stage launch");
     }
@@ -627,44 +648,53 @@ static void advance(lgf_batch* b, int stages) {
     }
 }
 
+// Each cache group runs all nsteps before the next group starts: fields are
+// independent, so the order of groups changes no result, and a group whose
+// links, next links and accumulator fit the last-level cache keeps them
+// there across its stages instead of streaming the whole batch from HBM
+// every stage.
 extern "C" int lgf_batch_flow_steps(lgf_batch* b, const double* eps, int nsteps, int use_graph) {
     if (!b || !eps || nsteps < 0) return LGF_E_ARG;
     lgf_ctx* ctx = b->ctx;
     int rc = batch_sync_desc(b, eps);
     if (rc) return rc;
     const int var = ctx->stage_variant[b->prec];
-    while (nsteps > 0) {
-        if (use_graph && nsteps >= 2) {
-            const int g = b->ping;
-            if (b->graph[g] && b->graph_variant[g] != var) {
-                (void)hipGraphExecDestroy(b->graph[g]);
-                b->graph[g] = nullptr;
+    for (size_t gi = 0; gi < b->groups.size(); ++gi) {
+        const int t0 = b->groups[gi].x, nt = b->groups[gi].y;
+        int p = b->ping, n = nsteps;
+        while (n > 0) {
+            if (use_graph && n >= 2) {
+                hipGraphExec_t& gx = b->graph[2 * gi + p];
+                int& gv = b->graph_variant[2 * gi + p];
+                if (gx && gv != var) {
+                    (void)hipGraphExecDestroy(gx);
+                    gx = nullptr;
+                }
+                if (!gx) {
+                    hipGraph_t graph;
+                    HIPTRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+                    int rc1 = enqueue_step(b, p, t0, nt);
+                    int rc2 = enqueue_step(b, p ^ 1, t0, nt);
+                    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+                    if (rc1 || rc2) return rc1 ? rc1 : rc2;
+                    if (e != hipSuccess) return lgf_hip_fail(ctx, e, "graph capture");
+                    e = hipGraphInstantiate(&gx, graph, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(graph);
+                    if (e != hipSuccess) return lgf_hip_fail(ctx, e, "graph instantiate");
+                    gv = var;
+                }
+                HIPTRY(ctx, hipGraphLaunch(gx, ctx->stream));
+                n -= 2;  // six stages: the group is back at ping p
+            } else {
+                rc = enqueue_step(b, p, t0, nt);
+                if (rc) return rc;
+                p ^= 1;
+                n -= 1;
             }
-            if (!b->graph[g]) {
-                hipGraph_t graph;
-                HIPTRY(ctx, hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
-                int rc1 = enqueue_step(b, g);
-                int rc2 = enqueue_step(b, g ^ 1);
-                hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
-                if (rc1 || rc2) return rc1 ? rc1 : rc2;
-                if (e != hipSuccess) return lgf_hip_fail(ctx, e, "graph capture");
-                e = hipGraphInstantiate(&b->graph[g], graph, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(graph);
-                if (e != hipSuccess) return lgf_hip_fail(ctx, e, "graph instantiate");
-                b->graph_variant[g] = var;
-            }
-            HIPTRY(ctx, hipGraphLaunch(b->graph[g], ctx->stream));
-            advance(b, 6);
-            add_flow_time(b, eps, 2);
-            nsteps -= 2;
-        } else {
-            rc = enqueue_step(b, b->ping);
-            if (rc) return rc;
-            advance(b, 3);
-            add_flow_time(b, eps, 1);
-            nsteps -= 1;
         }
     }
+    advance(b, 3 * nsteps);
+    add_flow_time(b, eps, nsteps);
     return LGF_OK;
 }
 
@@ -893,9 +923,10 @@ extern "C" int lgf_measure_finish(lgf_field* f, double* obs) {
 // each candidate is timed kAutotuneReps times and its fastest run counts
 constexpr int kAutotuneReps = 3;
 
-static int time_steps(lgf_ctx* ctx, lgf_batch* b, double eps, int nsteps, hipEvent_t e0, hipEvent_t e1, float* ms) {
+static int time_steps(lgf_ctx* ctx, lgf_batch* b, const double* eps, int nsteps, hipEvent_t e0, hipEvent_t e1,
+                      float* ms) {
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return lgf_fail(ctx, LGF_E_HIP, "event record");
-    int rc = lgf_batch_flow_steps(b, &eps, nsteps, 0);
+    int rc = lgf_batch_flow_steps(b, eps, nsteps, 0);
     if (rc) return rc;
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(ms, e0, e1) != hipSuccess)
@@ -903,12 +934,39 @@ static int time_steps(lgf_ctx* ctx, lgf_batch* b, double eps, int nsteps, hipEve
     return LGF_OK;
 }
 
+// fastest of kAutotuneReps timings of nsteps steps (after one untimed step)
+// of a batch of `fields` created with the context's current settings
+static int time_batch(lgf_ctx* ctx, const std::vector<lgf_field*>& fields, const std::vector<double>& eps,
+                      int nsteps, hipEvent_t e0, hipEvent_t e1, double* ms_step) {
+    lgf_batch* b = nullptr;
+    int rc = lgf_batch_create(ctx, fields.data(), (int)fields.size(), &b);
+    if (!rc) rc = lgf_batch_flow_steps(b, eps.data(), 1, 0);
+    *ms_step = 1e300;
+    for (int r = 0; r < kAutotuneReps && !rc; ++r) {
+        float m = 0.f;
+        rc = time_steps(ctx, b, eps.data(), nsteps, e0, e1, &m);
+        if (!rc) *ms_step = std::min(*ms_step, (double)m / nsteps);
+    }
+    if (b) lgf_batch_destroy(b);
+    return rc;
+}
+
+// Cache-group caps for batches of several lattices.  The (order, variant)
+// search runs with the batch as one group and with groups of kGroupSearch
+// bytes (stage kernels with non-temporal stores gain nothing from groups,
+// the others a quarter on batches that overflow the 256 MB last-level
+// cache, so the best pair differs); the cap is then refined at the best pair.
+static const size_t kGroupSearch = size_t(224) << 20;
+static const size_t kGroupCaps[] = {0, size_t(128) << 20, size_t(192) << 20, size_t(224) << 20,
+                                    size_t(256) << 20};
+
 static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int precision, int nsteps, int* best,
                            double* ms_per_step) {
     static const int orders[] = {0, 2, 4, 8, 404, 408};
     constexpr int norders = sizeof(orders) / sizeof(orders[0]);
     const int nv = num_stage_variants(precision);
     const int keep_order = ctx->tile_block, keep_variant = ctx->stage_variant[precision];
+    const size_t keep_groups = ctx->group_bytes;
     std::vector<lgf_field*> fields;
     int rc = LGF_OK;
     for (int i = 0; i < nshapes && !rc; ++i) {
@@ -922,22 +980,47 @@ static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int pre
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (!rc && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
         rc = lgf_fail(ctx, LGF_E_HIP, "event creation");
-    const double eps = 0.01;
-    std::vector<double> ms(size_t(norders) * nv, 1e300);
-    for (int oi = 0; oi < norders && !rc; ++oi) {
-        ctx->tile_block = orders[oi];
-        lgf_batch* b = nullptr;
-        rc = lgf_batch_create(ctx, fields.data(), (int)fields.size(), &b);
-        for (int v = 0; v < nv && !rc; ++v) {
-            ctx->stage_variant[precision] = v;
-            rc = lgf_batch_flow_steps(b, &eps, 1, 0);
-            for (int r = 0; r < kAutotuneReps && !rc; ++r) {
-                float m = 0.f;
-                rc = time_steps(ctx, b, eps, nsteps, e0, e1, &m);
-                if (!rc) ms[size_t(oi) * nv + v] = std::min(ms[size_t(oi) * nv + v], (double)m / nsteps);
+    const std::vector<double> eps(fields.size(), 0.01);
+    // tile order and stage variant, jointly, as one group and (several
+    // lattices) in cache groups
+    const int ncaps = fields.size() > 1 ? 2 : 1;
+    const size_t search_caps[2] = {0, kGroupSearch};
+    std::vector<double> ms(size_t(ncaps) * norders * nv, 1e300);
+    for (int ci = 0; ci < ncaps && !rc; ++ci) {
+        ctx->group_bytes = search_caps[ci];
+        for (int oi = 0; oi < norders && !rc; ++oi) {
+            ctx->tile_block = orders[oi];
+            for (int v = 0; v < nv && !rc; ++v) {
+                ctx->stage_variant[precision] = v;
+                rc = time_batch(ctx, fields, eps, nsteps, e0, e1, &ms[(size_t(ci) * norders + oi) * nv + v]);
             }
         }
-        if (b) lgf_batch_destroy(b);
+    }
+    int bo = 0, bv = 0, bc = 0;
+    size_t bg = keep_groups;
+    if (!rc) {
+        const size_t arg = size_t(std::min_element(ms.begin(), ms.end()) - ms.begin());
+        bc = int(arg / (size_t(norders) * nv));
+        bo = int(arg / nv % norders);
+        bv = int(arg % nv);
+        ctx->tile_block = orders[bo];
+        ctx->stage_variant[precision] = bv;
+        if (fields.size() > 1) {
+            // refine the cache-group cap at the best pair
+            double best_ms = ms[arg];
+            bg = search_caps[bc];
+            for (size_t cap : kGroupCaps) {
+                if (cap == search_caps[0] || cap == search_caps[1]) continue;
+                ctx->group_bytes = cap;
+                double m = 0;
+                rc = time_batch(ctx, fields, eps, nsteps, e0, e1, &m);
+                if (rc) break;
+                if (m < best_ms) {
+                    best_ms = m;
+                    bg = cap;
+                }
+            }
+        }
     }
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
@@ -945,15 +1028,27 @@ static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int pre
     if (rc) {
         ctx->tile_block = keep_order;
         ctx->stage_variant[precision] = keep_variant;
+        ctx->group_bytes = keep_groups;
         return rc;
     }
-    const size_t arg = size_t(std::min_element(ms.begin(), ms.end()) - ms.begin());
-    const int bo = int(arg / nv), bv = int(arg % nv);
     ctx->tile_block = orders[bo];
     ctx->stage_variant[precision] = bv;
+    ctx->group_bytes = bg;
     if (best) *best = bv;
     if (ms_per_step)
-        for (int v = 0; v < nv; ++v) ms_per_step[v] = ms[size_t(bo) * nv + v];
+        for (int v = 0; v < nv; ++v) ms_per_step[v] = ms[(size_t(bc) * norders + bo) * nv + v];
+    return LGF_OK;
+}
+
+extern "C" int lgf_ctx_set_batch_group_bytes(lgf_ctx* ctx, size_t bytes) {
+    if (!ctx) return LGF_E_ARG;
+    ctx->group_bytes = bytes;
+    return LGF_OK;
+}
+
+extern "C" int lgf_ctx_get_batch_group_bytes(lgf_ctx* ctx, size_t* bytes) {
+    if (!ctx || !bytes) return LGF_E_ARG;
+    *bytes = ctx->group_bytes;
     return LGF_OK;
 }
 

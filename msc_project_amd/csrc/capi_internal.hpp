@@ -21,6 +21,7 @@ struct lgf_ctx {
     void* lines = nullptr;    // line products of the clover measurement (grown on demand)
     size_t lines_bytes = 0;
     size_t lines_cap = size_t(1) << 30;  // fields are measured in chunks of about this many line bytes
+    size_t group_bytes = 0;   // cache groups of new batches (lgf_ctx_set_batch_group_bytes); 0 = one group
     int live_fields = 0;      // fields and user batches not yet destroyed
     int live_batches = 0;
 };
@@ -53,8 +54,13 @@ struct lgf_batch {
     double* d_out = nullptr;
     double* h_out = nullptr;
     int ping = 0;
-    hipGraphExec_t graph[2] = {nullptr, nullptr};  // two RK3 steps, by starting ping
-    int graph_variant[2] = {-1, -1};
+    // cache groups: consecutive fields whose footprint fits ctx->group_bytes,
+    // as (first tile, tiles) of the tile list; the flow runs each group's
+    // steps back to back
+    std::vector<int2> groups;
+    // per group, two RK3 steps captured as a graph, by starting ping
+    std::vector<hipGraphExec_t> graph;  // [2 * group + ping]
+    std::vector<int> graph_variant;
     bool is_solo = false;  // a field's private one-field batch
     // deferred measurement (lgf_batch_measure_start / _finish): own partials
     // and result buffers, completion event

@@ -1,94 +1,17 @@
-// exp(X) for X anti-Hermitian and traceless (an su(3) element), two forms.
-//
-// exp_ah_cayley: the closed Cayley-Hamilton form of Morningstar & Peardon,
-//   PRD 69 (2004) 054501, with exactly the branch structure of the reference
-//   HermTrLessExp (LGF/Math/SU3_mat.cpp:216-289): series for xi0 when
-//   |w| <= 0.05, Taylor coefficients when c1 <= 1e-4, and the c0 < 0
-//   conjugation symmetry.  Used by the FP64 parity path, where results must
-//   follow the reference's formula to 1e-12 over hundreds of steps.
-//
-// exp_ah_series: the same f0 + f1 Q + f2 Q^2 decomposition, but with the f_j
-//   summed from the exponential series using the Cayley-Hamilton recurrence
-//   Q^{n+1} = c0 d_n + (a_n + c1 d_n) Q + b_n Q^2 for Q^n = a_n + b_n Q + d_n Q^2.
-//   No transcendental functions; truncation below 1e-9 (see the bound at
-//   exp_ah_series).  Larger exponents are scaled and squared.  Used by the
-//   mixed-precision path.
+// exp(X) for X anti-Hermitian and traceless (an su(3) element), as
+// f0 + f1 Q + f2 Q^2 with Q = -iX (Cayley-Hamilton: every power of Q is a
+// combination of 1, Q, Q^2), the f_j summed from the exponential series with
+// the recurrence Q^{n+1} = c0 d_n + (a_n + c1 d_n) Q + b_n Q^2 for
+// Q^n = a_n + b_n Q + d_n Q^2 (c0 = det Q, c1 = Tr Q^2 / 2).  No
+// transcendental functions and no data-dependent branches but the rare
+// scaling of a large exponent.  The reference computes the same exponential
+// in closed form (HermTrLessExp, LGF/Math/SU3_mat.cpp:216-289); both are
+// accurate to rounding, and the FP64 flow with this series agrees with the
+// reference's to ~1e-14 in E over its golden runs.
 #pragma once
 #include "su3.hpp"
 
 namespace lgf {
-
-struct Cx {
-    double re, im;
-};
-
-// Closed form, double precision, reference branch structure.
-LGF_DEV M3<double> exp_ah_cayley(const M3<double>& X) {
-    M3<double> Q;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {  // Q = -i X
-        Q.v[2 * i] = X.v[2 * i + 1];
-        Q.v[2 * i + 1] = -X.v[2 * i];
-    }
-    M3<double> Q2 = mul_nn(Q, Q);
-    double c0 = retr_mul(Q, Q2) / 3.0;
-    double c1 = retr(Q2) / 2.0;
-    double c1_sqrt = sqrt(c1);
-    // clamped as the reference's _DEBUG build does (LGF/Math/SU3_mat.cpp:230-238);
-    // equal to the release build wherever that one is finite
-    double ratio = fmin(fabs(c0) / (2.0 * pow((c1 / 3.0), 1.5)), 1.0);
-    double theta3 = acos(ratio) / 3.0;
-    double u = sqrt(1.0 / 3.0) * c1_sqrt * cos(theta3);
-    double u2 = u * u;
-    double ucos = cos(u), usin = sin(u);
-    double w = c1_sqrt * sin(theta3);
-    double w2 = w * w;
-    double wcos = cos(w);
-    double xi0 = (fabs(w) > 0.05) ? sin(w) / w : 1.0 - w2 * (1.0 - w2 * (1 - w2 / 42.0) / 20.0) / 6.0;
-    Cx f0, f1, f2;
-    if (c1 > 0.0001) {
-        double div = 1.0 / (9.0 * u2 - w2);
-        Cx e2 = {(2.0 * ucos * ucos - 1.0), 2.0 * ucos * usin};
-        Cx em = {ucos, -usin};
-        Cx h = {8.0 * u2 * wcos, 2.0 * u * (3.0 * u2 + w2) * xi0};
-        double s = u2 - w2;
-        f0.re = div * (s * e2.re + (em.re * h.re - em.im * h.im));
-        f0.im = div * (s * e2.im + (em.re * h.im + em.im * h.re));
-        h = Cx{2.0 * u * wcos, (w2 - 3.0 * u2) * xi0};
-        double t = 2.0 * u;
-        f1.re = div * (t * e2.re - (em.re * h.re - em.im * h.im));
-        f1.im = div * (t * e2.im - (em.re * h.im + em.im * h.re));
-        h = Cx{wcos, 3.0 * u * xi0};
-        f2.re = div * (e2.re - (em.re * h.re - em.im * h.im));
-        f2.im = div * (e2.im - (em.re * h.im + em.im * h.re));
-        if (c0 < 0.0) {
-            f0.im = -f0.im;
-            f1.re = -f1.re;
-            f2.im = -f2.im;
-        }
-    } else {
-        double c0s = c0 * c0;
-        f0 = Cx{1.0 - c0s / 720.0, -c0 * (1.0 - c1 * (1 - c1 / 42.0) / 20.0) / 6.0};
-        f1 = Cx{c0 * (1.0 - c1 * (1.0 - 3.0 * c1 / 112.0) / 15.0) / 24.0,
-                1.0 - c1 * (1.0 - c1 * (1 - c1 / 42.0) / 20.0) / 6.0 - c0s / 5040.0};
-        f2 = Cx{(-1.0 + c1 * (1.0 - c1 * (1.0 - c1 / 56.0) / 30.0) / 12.0 + c0s / 20160.0) / 2.0,
-                (c0 * (1.0 - c1 * (1.0 - c1 / 48.0) / 21.0) / 60.0) / 2.0};
-    }
-    M3<double> o;
-#pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        double qr = Q.v[2 * i], qi = Q.v[2 * i + 1], pr = Q2.v[2 * i], pi = Q2.v[2 * i + 1];
-        o.v[2 * i] = (qr * f1.re - qi * f1.im) + (pr * f2.re - pi * f2.im);
-        o.v[2 * i + 1] = (qr * f1.im + qi * f1.re) + (pr * f2.im + pi * f2.re);
-    }
-    o.v[0] += f0.re;
-    o.v[1] += f0.im;
-    o.v[8] += f0.re;
-    o.v[9] += f0.im;
-    o.v[16] += f0.re;
-    o.v[17] += f0.im;
-    return o;
-}
 
 // Q^2 and Re Tr(Q^3) for Hermitian Q (the exponent's -iX, X anti-Hermitian
 // with exactly mirrored lower triangle): Q^2 is Hermitian with a real
@@ -140,11 +63,11 @@ LGF_DEV void herm_square(const M3<R>& Q, M3<R>& Q2, R& tr3) {
 // Power series through the Cayley-Hamilton recurrence, real type R.
 // Truncated after Q^(NTERMS-1): with c1 <= C1MAX every eigenvalue satisfies
 // |lambda| <= sqrt(4 c1 / 3) and the first omitted term is below
-// |lambda|^NTERMS / NTERMS!.  The defaults (7 terms, c1 <= 0.02, so
-// |lambda| <= 0.164) bound it by 7e-10, far below FP32 rounding; a flow
-// step's exponent eps*Z has c1 ~ 1e-4..1e-3 at eps = 0.01.  Larger
-// exponents are scaled by 2^-k into that range and the result squared k
-// times, all in precision R.
+// |lambda|^NTERMS / NTERMS!.  With c1 <= 0.02 (|lambda| <= 0.164) that is
+// 7e-10 for the default 7 terms, far below FP32 rounding, and 1e-20 for the
+// 13 terms of the FP64 path; a flow step's exponent eps*Z has c1 ~
+// 1e-4..1e-3 at eps = 0.01.  Larger exponents are scaled by 2^-k into that
+// range and the result squared k times, all in precision R.
 template <typename R, int NTERMS = 7>
 LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
     constexpr float C1MAX = 0.02f;

@@ -67,12 +67,9 @@ LGF_DEV void stage_link(const S* __restrict__ U, S* __restrict__ Un, S* __restri
         for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
     }
 
-    M3<C> ex;
-    if constexpr (EXPMODE == 0) {
-        ex = exp_ah_cayley(E);  // C == double
-    } else {
-        ex = exp_ah_series<C>(E);
-    }
+    // the exponential by its power series: 13 terms in the FP64 path
+    // (truncation below 1e-20), 7 in the mixed-precision one (below 1e-9)
+    const M3<C> ex = exp_ah_series<C, EXPMODE == 0 ? 13 : 7>(E);
     M3<C> Unew = su3_nn(ex, Umu);
     if constexpr (EXPMODE == 1 && STAGE == 2) {
         // once per RK3 step: project back onto SU(3) in double precision

@@ -77,6 +77,8 @@ SIGNATURES = {
     "lgf_autotune_batch": ([_vp, _ip, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip, _dp], ctypes.c_int),
     "lgf_ctx_set_tile_order": ([_vp, ctypes.c_int], ctypes.c_int),
     "lgf_ctx_get_tile_order": ([_vp, _ip], ctypes.c_int),
+    "lgf_ctx_set_batch_group_bytes": ([_vp, ctypes.c_size_t], ctypes.c_int),
+    "lgf_ctx_get_batch_group_bytes": ([_vp, ctypes.POINTER(ctypes.c_size_t)], ctypes.c_int),
     "lgf_ctx_set_measure_scratch": ([_vp, ctypes.c_size_t], ctypes.c_int),
     "lgf_measure_observables": ([_vp, ctypes.c_int, _dp, _dp], ctypes.c_int),
     "lgf_measure_observables_noalloc": ([_vp, ctypes.c_int, _dp], ctypes.c_int),
@@ -224,6 +226,15 @@ class Context:
     def tile_order(self):
         v = ctypes.c_int()
         self.check(lib().lgf_ctx_get_tile_order(self.h, ctypes.byref(v)))
+        return v.value
+
+    def set_batch_group_bytes(self, nbytes):
+        """cache-group cap of batches created afterwards (0: one group)"""
+        self.check(lib().lgf_ctx_set_batch_group_bytes(self.h, int(nbytes)))
+
+    def batch_group_bytes(self):
+        v = ctypes.c_size_t()
+        self.check(lib().lgf_ctx_get_batch_group_bytes(self.h, ctypes.byref(v)))
         return v.value
 
     def stage_variant(self, precision):

@@ -181,8 +181,13 @@ def test_observables_match_oracle(ctx, shape):
         f.close()
 
 
-def test_batch_same_alone_and_in_mix(ctx):
-    """each lattice's results are bitwise the same flowed alone or batched with others"""
+@pytest.mark.parametrize("groups,use_graph", [("one", False), ("one", True), ("each", False), ("each", True),
+                                              ("pairs", True)])
+def test_batch_same_alone_and_in_mix(ctx, groups, use_graph):
+    """each lattice's results are bitwise the same flowed alone or batched
+    with others, whether the batch runs as one group or in cache groups
+    (every field alone, or two fields per group), with graphs or direct
+    launches, over an odd number of steps"""
     shapes = [(4, 4, 4, 4), (6, 4, 4, 8), (4, 4, 4, 4), (8, 6, 4, 4)]
     eps = [0.02, 0.01, 0.03, 0.02]
     inputs = [O.generate(s, 6.0, seed=100 + i, start="hot", sweeps=2) for i, s in enumerate(shapes)]
@@ -192,16 +197,24 @@ def test_batch_same_alone_and_in_mix(ctx):
             f = lgf.Field(ctx, s, prec)
             f.upload(U)
             f.flow(e, 5)
-            alone.append((f.download(), f.measure()[:3]))
+            alone.append((f.download(), f.measure()[:3], f.flow_time))
+        cap = {"one": 0, "each": 1, "pairs": lgf.footprint(shapes[0], prec) + lgf.footprint(shapes[1], prec)}[groups]
+        ctx.set_batch_group_bytes(cap)
         fields = [lgf.Field(ctx, s, prec) for s in shapes]
         for f, U in zip(fields, inputs):
             f.upload(U)
         b = lgf.Batch(ctx, fields)
-        b.flow(eps, 5, use_graph=False)
+        ctx.set_batch_group_bytes(0)  # groups are fixed when the batch is created
+        b.flow(eps, 2, use_graph=use_graph)
+        b.flow(eps, 3, use_graph=use_graph)
         E, Q, Qa = b.measure()
         for i, f in enumerate(fields):
             assert np.array_equal(f.download(), alone[i][0])
             assert (E[i], Q[i], Qa[i]) == alone[i][1]
+            assert f.flow_time == alone[i][2]
+        b.close()
+        for f in fields:
+            f.close()
 
 
 def test_measure_in_groups_matches_one_launch(ctx):
