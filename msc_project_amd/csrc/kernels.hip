@@ -43,8 +43,10 @@ namespace lgf {
 template <typename S, typename C, int STAGE, int EXPMODE, int MU>
 LGF_DEV void stage_link(const S* __restrict__ U, S* __restrict__ Un, S* __restrict__ Xa, int64_t st, const Geo& g,
                         C eps) {
-    const int s = g.s;
     const M3<C> A = staple_ct<S, C, MU>(U, st, g);
+    // the link and the accumulator are loaded after the staple sum (their
+    // addresses depend on it), so they do not hold registers through it
+    const int s = after_value(g.s, A.v[17]);
     const M3<C> Umu = load_link<S, C>(U, st, MU, s);
     const M3<C> Z = ta_of_product(Umu, A, C(-0.5));
 
@@ -54,7 +56,7 @@ LGF_DEV void stage_link(const S* __restrict__ U, S* __restrict__ Un, S* __restri
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = (C(0.25) * eps) * Z.v[i];
     } else if (STAGE == 1) {
-        const M3<C> X0 = load_ta<S, C>(Xa, st, MU, s);
+        const M3<C> X0 = load_ta<S, C>(Xa, st, MU, after_value(s, Z.v[17]));
         M3<C> X1;
 #pragma unroll
         for (int i = 0; i < 18; ++i) X1.v[i] = (C)(8.0 / 9.0) * Z.v[i] - (C)(17.0 / 36.0) * X0.v[i];
@@ -62,7 +64,7 @@ LGF_DEV void stage_link(const S* __restrict__ U, S* __restrict__ Un, S* __restri
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = eps * X1.v[i];
     } else {
-        const M3<C> X1 = load_ta<S, C>(Xa, st, MU, s);
+        const M3<C> X1 = load_ta<S, C>(Xa, st, MU, after_value(s, Z.v[17]));
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
     }

@@ -77,17 +77,46 @@ LGF_DEV void staple_term(M3<C>& A, const S* __restrict__ U, int64_t st, const Ge
     lower_term<S, C, MU, NU>(A, U, st, g, fmu);
 }
 
+// Site index s passed through an asm that takes `dep`: loads addressed from
+// the result cannot be issued before dep is computed, so the links of a
+// chain of products are in flight one step at a time.
+template <typename C>
+LGF_DEV int after_value(int s, C dep) {
+    asm volatile("" : "+v"(s) : "v"(dep));
+    return s;
+}
+
+// The staple term of the (MU,NU) plane as chains of products with the third
+// link of each half loaded after the first product:
+//   upper  U_nu(x+mu) (U_nu(x) U_mu(x+nu))^dag = (U_nu(x+mu) U_mu(x+nu)^dag) U_nu(x)^dag
+//   lower  (U_mu(x-nu) U_nu(x-nu+mu))^dag U_nu(x-nu) = U_nu(x-nu+mu)^dag (U_mu(x-nu)^dag U_nu(x-nu))
+// The same products as staple_term regrouped, so at most two links and one
+// partial product are live besides the sum: an FP64 matrix less at the
+// register peak.  Terms follow each other the same way.
+template <typename S, typename C, int MU, int NU>
+LGF_DEV void staple_term_chain(M3<C>& A, const S* __restrict__ U, int64_t st, const Geo& g, int fmu) {
+    const int s = after_value(g.s, A.v[17]);
+    const int fnu = nb_off(g, NU, +1), bnu = nb_off(g, NU, -1);
+    {
+        const M3<C> M = su3_nd(load_link<S, C>(U, st, NU, s + fmu), load_link<S, C>(U, st, MU, s + fnu));
+        mul_nd_acc(A, M, load_link<S, C>(U, st, NU, after_value(s, M.v[17])));
+    }
+    {
+        const int s2 = after_value(s, A.v[17]);
+        const M3<C> M = su3_dn(load_link<S, C>(U, st, MU, s2 + bnu), load_link<S, C>(U, st, NU, s2 + bnu));
+        mul_dn_acc(A, load_link<S, C>(U, st, NU, after_value(s2, M.v[17]) + bnu + fmu), M);
+    }
+}
+
 template <typename S, typename C, int MU>
 LGF_DEV M3<C> staple_ct(const S* __restrict__ U, int64_t st, const Geo& g) {
     constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
     const int fmu = nb_off(g, MU, +1);
     M3<C> A;
     set_zero(A);
-    staple_term<S, C, MU, N0>(A, U, st, g, fmu);
-    asm volatile("" ::: "memory");  // one term's loads at a time: bounds the live registers
-    staple_term<S, C, MU, N1>(A, U, st, g, fmu);
-    asm volatile("" ::: "memory");
-    staple_term<S, C, MU, N2>(A, U, st, g, fmu);
+    staple_term_chain<S, C, MU, N0>(A, U, st, g, fmu);
+    staple_term_chain<S, C, MU, N1>(A, U, st, g, fmu);
+    staple_term_chain<S, C, MU, N2>(A, U, st, g, fmu);
     return A;
 }
 

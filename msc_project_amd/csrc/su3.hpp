@@ -265,6 +265,27 @@ LGF_DEV M3<R> mul_nd_su3(const M3<R>& a, const M3<R>& b) {
     return o;
 }
 
+// a^dag b from rows 0, 1 and their cross product
+template <typename R>
+LGF_DEV M3<R> mul_dn_su3(const M3<R>& a, const M3<R>& b) {
+    M3<R> o;
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            R x = R(0), y = R(0);
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                LGF_CMAC(x, y, a.v[2 * (3 * k + r)], -a.v[2 * (3 * k + r) + 1], b.v[2 * (3 * k + c)],
+                         b.v[2 * (3 * k + c) + 1])
+            }
+            o.v[2 * (3 * r + c)] = x;
+            o.v[2 * (3 * r + c) + 1] = y;
+        }
+    su3_third_row(o);
+    return o;
+}
+
 // Products of SU(3) matrices in the arithmetic of the mixed-precision path
 // (float: links projected onto SU(3) every flow step) use the cross-product
 // row; in double, the parity path, the full product as the reference.
@@ -282,6 +303,14 @@ LGF_DEV M3<R> su3_nd(const M3<R>& a, const M3<R>& b) {
         return mul_nd_su3(a, b);
     else
         return mul_nd(a, b);
+}
+
+template <typename R>
+LGF_DEV M3<R> su3_dn(const M3<R>& a, const M3<R>& b) {
+    if constexpr (sizeof(R) == 4)
+        return mul_dn_su3(a, b);
+    else
+        return mul_dn(a, b);
 }
 
 // ta_scaled(a * b, s) without the real parts of the product's diagonal,
