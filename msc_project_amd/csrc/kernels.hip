@@ -517,6 +517,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 #ifndef LGF_CLOVER_WPE
 #define LGF_CLOVER_WPE 4
 #endif
+//   LGF_CLOVER_WPE64    the same for FP64 arithmetic on FP32 links (the
+//                       mixed-precision path's final sweep) and on FP64 links
+#ifndef LGF_CLOVER_WPE64
+#define LGF_CLOVER_WPE64 3
+#endif
+#ifndef LGF_CLOVER_WPE64_PARITY
+#define LGF_CLOVER_WPE64_PARITY 2
+#endif
 #ifndef LGF_CLOVER_LDS_REAL
 #define LGF_CLOVER_LDS_REAL 1
 #endif
@@ -560,11 +568,38 @@ struct Lines {
     int lbase;    // line index of site 0 of this field
 };
 
-template <int LEN, typename S, typename C>
+// Products of lines.  With FP32 links (the mixed-precision path, whose links
+// are projected onto SU(3) every flow step) they are SU(3) products: rows 0
+// and 1 and their conjugated cross product.  A line that is only ever a left
+// factor is then read as rows 0 and 1 (ROWS = 2), a right factor whole
+// (ROWS = 3).  In the FP64-arithmetic sweep over FP32 links (kCloverRows) a
+// right factor too is read as two rows and its third rebuilt: that sweep is
+// bound by the bytes its lines move through the vector memory path (VALU
+// 30-40 % busy), and a rebuilt row costs 24 FMAs against 48 of 144 bytes.
+// The FP32 sweep is bound by its arithmetic and reads right factors whole.
+// With FP64 links (the parity path) the products are full, as the
+// reference's: its links are not projected, and rebuilding rows would move
+// the result by the links' drift from SU(3).
+#ifndef LGF_CLOVER_ROWS2
+#define LGF_CLOVER_ROWS2 1
+#endif
+template <typename S>
+constexpr bool kCloverSu3 = sizeof(S) == 4;
+template <typename S, typename C>
+constexpr bool kCloverRows = LGF_CLOVER_ROWS2 && kCloverSu3<S> && sizeof(C) == 8;
+
+template <int LEN, int ROWS, typename S, typename C>
 LGF_DEV M3<C> line(const Lines<S, C>& L, int dir, int site) {
-    if constexpr (LEN == 1) return load_link<S, C>(L.U, L.st, dir, site);
-    else if constexpr (LEN == 2) return load_link<C, C>(L.L2, L.G, dir, L.lbase + site);
-    else return load_link<C, C>(L.L3, L.G, dir, L.lbase + site);
+    if constexpr (kCloverSu3<S> && (ROWS == 2 || kCloverRows<S, C>)) {
+        M3<C> m = LEN == 1 ? load_link_rows01<S, C>(L.U, L.st, dir, site)
+                           : load_link_rows01<C, C>(LEN == 2 ? L.L2 : L.L3, L.G, dir, L.lbase + site);
+        if constexpr (ROWS == 3) su3_third_row(m);
+        return m;
+    } else if constexpr (LEN == 1) {
+        return load_link<S, C>(L.U, L.st, dir, site);
+    } else {
+        return load_link<C, C>(LEN == 2 ? L.L2 : L.L3, L.G, dir, L.lbase + site);
+    }
 }
 
 // site offset of x + steps*dir (|steps| <= 3, periodic); rcp = the field's
@@ -589,21 +624,41 @@ LGF_DEV void scale(M3<C>& a, C w) {
     for (int i = 0; i < 18; ++i) a.v[i] *= w;
 }
 
+template <typename S, typename C>
+LGF_DEV M3<C> clover_nn(const M3<C>& a, const M3<C>& b) {
+    if constexpr (kCloverSu3<S>) return mul_nn_su3(a, b);
+    else return mul_nn(a, b);
+}
+template <typename S, typename C>
+LGF_DEV M3<C> clover_nd(const M3<C>& a, const M3<C>& b) {
+    if constexpr (kCloverSu3<S>) return mul_nd_su3(a, b);
+    else return mul_nd(a, b);
+}
+
+// In FP64 arithmetic (kCloverChain) a leaf's products are chained: the
+// lines of each product are loaded after the previous product is formed, so
+// one product's operands are live at a time.  The FP32 kernel keeps a
+// leaf's loads together (more in flight) and fits its budget as it is.
+template <typename C>
+constexpr bool kCloverChain = sizeof(C) == 8;
+
 // diagonal quadrants: it = 0 is (+nu,+mu), it = 1 is (-nu,-mu)
 template <int M, int N, typename S, typename C>
-LGF_DEV void diag_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
+LGF_DEV void diag_leaf(Ta9<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
 #if LGF_CLOVER_FENCE
     asm volatile("" ::: "memory");  // keep the next leaf's loads from being hoisted into this one
 #endif
 #if LGF_CLOVER_ORDERED
-    asm volatile("" : "+v"(s) : "v"(acc.v[17]));
+    asm volatile("" : "+v"(s) : "v"(acc.v[8]));
 #endif
     const int mb = o.m[3 - M], nb = o.n[3 - N], mf = o.m[3 + M], nf = o.n[3 + N];
     const int sc = it ? s + nb + mb : s;
     const int scn = it ? s + mb : s + nf;  // c + n nu
-    const int scm = it ? s + nb : s + mf;  // c + m mu
-    M3<C> P = su3_nn(line<N>(L, nu, sc), line<M>(L, mu, scn));
-    const M3<C> R = su3_nn(line<M>(L, mu, sc), line<N>(L, nu, scm));
+    M3<C> P = clover_nn<S, C>(line<N, 2>(L, nu, sc), line<M, 3>(L, mu, scn));
+    const int s2 = kCloverChain<C> ? after_value(s, P.v[17]) : s;
+    const int sc2 = it ? s2 + nb + mb : s2;
+    const int scm = it ? s2 + nb : s2 + mf;  // c + m mu
+    const M3<C> R = clover_nn<S, C>(line<M, 2>(L, mu, sc2), line<N, 3>(L, nu, scm));
     if (w != C(1)) scale(P, w);
     // the clover sums enter only through their traceless anti-Hermitian part
     if (it == 0) mul_nd_acc_ta(acc, P, R);
@@ -613,32 +668,45 @@ LGF_DEV void diag_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const Plane
 // side quadrants: it = 0 is (+mu,-nu) with (alpha, beta) = (mu, nu); it = 1
 // is (-mu,+nu) with (alpha, beta) = (nu, mu), entering as -W^dag
 template <int L1, int L2, typename S, typename C>
-LGF_DEV void side_leaf(M3<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
+LGF_DEV void side_leaf(Ta9<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
 #if LGF_CLOVER_FENCE
     asm volatile("" ::: "memory");
 #endif
 #if LGF_CLOVER_ORDERED
-    asm volatile("" : "+v"(s) : "v"(acc.v[17]));
+    asm volatile("" : "+v"(s) : "v"(acc.v[8]));
 #endif
     const int a = it ? nu : mu, b = it ? mu : nu;
     const int m2 = o.m[3 - L2], n2 = o.n[3 - L2], m1 = o.m[3 + L1], n1 = o.n[3 + L1];
     const int zb = it ? m2 : n2;  // z = x - l2 beta
     const int za = it ? n1 : m1;  // + l1 alpha
-    const M3<C> H = su3_nn(line<L1>(L, a, s + zb), line<L2>(L, b, s + zb + za));
-    M3<C> T = su3_nd(line<L1>(L, a, s), H);
+    const M3<C> H = clover_nn<S, C>(line<L1, 2>(L, a, s + zb), line<L2, 3>(L, b, s + zb + za));
+    const int s2 = kCloverChain<C> ? after_value(s, H.v[17]) : s;
+    M3<C> T = clover_nd<S, C>(line<L1, 2>(L, a, s2), H);
     scale(T, it ? -w : w);
-    mul_nn_acc_ta(acc, T, line<L2>(L, b, s + zb));
+    const int s3 = kCloverChain<C> ? after_value(s2, T.v[17]) : s2;
+    mul_nn_acc_ta(acc, T, line<L2, 3>(L, b, s3 + zb));
 }
 
 // F = 1/2 (-i) [ (C - C^dag) - 1/3 Tr(C - C^dag) ] (LGF/Field/SU3_field.cpp:575-579)
+// from the folded sum: the diagonal of C - C^dag is 2i t[d], so
+// F_dd = t[d] - (t[0] + t[1] + t[2]) / 3, and above it F_rc = -i/2 (C - C^dag)_rc.
 template <typename C>
-LGF_DEV M3<C> project_F(const M3<C>& Cl) {
-    M3<C> T = ta_scaled(Cl, C(1));
+LGF_DEV M3<C> project_F(const Ta9<C>& t) {
     M3<C> F;
+    const C m = (t.v[0] + t.v[1] + t.v[2]) * (C(1) / C(3));
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        F.v[2 * i] = C(0.5) * T.v[2 * i + 1];
-        F.v[2 * i + 1] = C(-0.5) * T.v[2 * i];
+    for (int d = 0; d < 3; ++d) {
+        F.v[8 * d] = t.v[d] - m;
+        F.v[8 * d + 1] = C(0);
+    }
+#pragma unroll
+    for (int p = 0; p < 3; ++p) {
+        const int r = p == 2 ? 1 : 0, c = p == 0 ? 1 : 2;
+        const C re = C(0.5) * t.v[4 + 2 * p], im = C(-0.5) * t.v[3 + 2 * p];
+        F.v[2 * (3 * r + c)] = re;
+        F.v[2 * (3 * r + c) + 1] = im;
+        F.v[2 * (3 * c + r)] = re;
+        F.v[2 * (3 * c + r) + 1] = -im;
     }
     return F;
 }
@@ -662,7 +730,7 @@ LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, const double* rcp, int mu,
     }
     const int s = g.s;
     const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
-    M3<C> acc;
+    Ta9<C> acc;
     set_zero(acc);
 #pragma unroll 1
     for (int it = 0; it < 2; ++it) diag_leaf<1, 1>(acc, C(1), L, s, o, mu, nu, it);
@@ -744,11 +812,23 @@ __global__ __launch_bounds__(256) void k_lines(const FieldDesc* __restrict__ fds
     if (s >= d.vol) return;
     const S* __restrict__ U = (const S*)d.links[ping ^ d.flip];
     const Geo g = make_geo(d, s);
-    const M3<C> L2 = su3_nn(load_link<S, C>(U, d.stride, mu, s), load_link<S, C>(U, d.stride, mu, s + nb_off(g, mu, +1)));
     const int li = (d.tile0 - t0) * kTileSites + s;
-    store_link<C, C>(lines, G, mu, li, L2);
-    store_link<C, C>(lines + 4 * 18 * G, G, mu, li,
-                     su3_nn(L2, load_link<S, C>(U, d.stride, mu, s + wrap_offset(g, mu, 2, d.rcp))));
+    if constexpr (kCloverRows<S, C>) {
+        // the clover reads these lines as rows 0 and 1 only
+        M3<C> U1 = load_link_rows01<S, C>(U, d.stride, mu, s + nb_off(g, mu, +1));
+        su3_third_row(U1);
+        const M3<C> L2 = mul_nn_su3(load_link_rows01<S, C>(U, d.stride, mu, s), U1);
+        store_link_rows01<C, C>(lines, G, mu, li, L2);
+        M3<C> U2 = load_link_rows01<S, C>(U, d.stride, mu, s + wrap_offset(g, mu, 2, d.rcp));
+        su3_third_row(U2);
+        store_link_rows01<C, C>(lines + 4 * 18 * G, G, mu, li, mul_nn_su3(L2, U2));
+    } else {
+        const M3<C> L2 =
+            su3_nn(load_link<S, C>(U, d.stride, mu, s), load_link<S, C>(U, d.stride, mu, s + nb_off(g, mu, +1)));
+        store_link<C, C>(lines, G, mu, li, L2);
+        store_link<C, C>(lines + 4 * 18 * G, G, mu, li,
+                         su3_nn(L2, load_link<S, C>(U, d.stride, mu, s + wrap_offset(g, mu, 2, d.rcp))));
+    }
 }
 
 // One thread per (site, plane).  The six planes are taken in the order of
@@ -764,8 +844,8 @@ __global__ __launch_bounds__(256) void k_lines(const FieldDesc* __restrict__ fds
 // Tiles write the kCloverParts partials (e, q, |q|, ei, q11) and, when
 // requested, q(x).  Reads the line products of the same chunk from k_lines.
 
-template <typename S, typename C>
-__global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(LGF_CLOVER_WPE, 8))) void k_clover(
+template <typename S, typename C, int WPE>
+__global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_clover(
     const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int t0, int ping, const C* __restrict__ lines,
     int64_t G, double* __restrict__ partials) {
     // F values are computed in C; keeping them as C in LDS loses nothing
@@ -1148,15 +1228,15 @@ hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2
     dim3 grid(ntiles), lblock(kTileSites, 4), block(kTileSites, 6);
     if (prec == kPrecFP64) {
         hipLaunchKernelGGL((k_lines<double, double>), grid, lblock, 0, stream, fds, ct, t0, ping, (double*)lines, G);
-        hipLaunchKernelGGL((k_clover<double, double>), grid, block, 0, stream, fds, ct, t0, ping,
+        hipLaunchKernelGGL((k_clover<double, double, LGF_CLOVER_WPE64_PARITY>), grid, block, 0, stream, fds, ct, t0, ping,
                            (const double*)lines, G, partials);
     } else if (compute == kPrecFP64) {
         hipLaunchKernelGGL((k_lines<float, double>), grid, lblock, 0, stream, fds, ct, t0, ping, (double*)lines, G);
-        hipLaunchKernelGGL((k_clover<float, double>), grid, block, 0, stream, fds, ct, t0, ping,
+        hipLaunchKernelGGL((k_clover<float, double, LGF_CLOVER_WPE64>), grid, block, 0, stream, fds, ct, t0, ping,
                            (const double*)lines, G, partials);
     } else {
         hipLaunchKernelGGL((k_lines<float, float>), grid, lblock, 0, stream, fds, ct, t0, ping, (float*)lines, G);
-        hipLaunchKernelGGL((k_clover<float, float>), grid, block, 0, stream, fds, ct, t0, ping, (const float*)lines,
+        hipLaunchKernelGGL((k_clover<float, float, LGF_CLOVER_WPE>), grid, block, 0, stream, fds, ct, t0, ping, (const float*)lines,
                            G, partials);
     }
     return hipGetLastError();

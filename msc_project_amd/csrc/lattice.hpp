@@ -163,6 +163,51 @@ LGF_DEV M3<C> load_link(const S* __restrict__ base, int64_t stride, int mu, int 
     return m;
 }
 
+// Rows 0 and 1 of a link (the first 12 reals, 2/3 of the bytes); the
+// caller rebuilds row 2 where it needs it (su3_third_row, exact for SU(3)).
+template <typename S, typename C>
+LGF_DEV M3<C> load_link_rows01(const S* __restrict__ base, int64_t stride, int mu, int s) {
+    M3<C> m;
+    const S* p = base + (int64_t)mu * 18 * stride;
+    if constexpr (sizeof(S) == 4) {
+        const uint32_t o4 = (uint32_t)s * 16u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const vf4 v = ld<vf4>(p + (int64_t)k * 4 * stride, o4);
+            m.v[4 * k] = (C)v.x;
+            m.v[4 * k + 1] = (C)v.y;
+            m.v[4 * k + 2] = (C)v.z;
+            m.v[4 * k + 3] = (C)v.w;
+        }
+    } else {
+        const uint32_t o = (uint32_t)s * 16u;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) {
+            const vd2 v = ld<vd2>(p + (int64_t)k * 2 * stride, o);
+            m.v[2 * k] = (C)v.x;
+            m.v[2 * k + 1] = (C)v.y;
+        }
+    }
+    return m;
+}
+
+template <typename S, typename C>
+LGF_DEV void store_link_rows01(S* __restrict__ base, int64_t stride, int mu, int s, const M3<C>& m) {
+    S* p = base + (int64_t)mu * 18 * stride;
+    if constexpr (sizeof(S) == 4) {
+        const uint32_t o4 = (uint32_t)s * 16u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            st_(p + (int64_t)k * 4 * stride, o4,
+                vf4{(float)m.v[4 * k], (float)m.v[4 * k + 1], (float)m.v[4 * k + 2], (float)m.v[4 * k + 3]});
+    } else {
+        const uint32_t o = (uint32_t)s * 16u;
+#pragma unroll
+        for (int k = 0; k < 6; ++k)
+            st_(p + (int64_t)k * 2 * stride, o, vd2{(double)m.v[2 * k], (double)m.v[2 * k + 1]});
+    }
+}
+
 template <typename S, typename C, bool NT = false>
 LGF_DEV void store_link(S* __restrict__ base, int64_t stride, int mu, int s, const M3<C>& m) {
     S* p = base + (int64_t)mu * 18 * stride;

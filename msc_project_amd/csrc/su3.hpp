@@ -207,6 +207,79 @@ LGF_DEV void mul_dn_acc_ta(M3<R>& o, const M3<R>& a, const M3<R>& b) {
 }
 #undef LGF_TA_ACC_BODY
 
+// A sum whose only use is its traceless anti-Hermitian part, folded to nine
+// reals as it is accumulated: v[d] = sum Im W_dd (d = 0..2) and, for the
+// pairs (r,c) = (0,1), (0,2), (1,2), v[3 + 2p], v[4 + 2p] = Re, Im of
+// sum (W_rc - conj(W_cr)) -- the upper triangle of W - W^dag.  Each product
+// term goes into one running sum, so this costs the FMAs of the fifteen
+// entries and holds nine registers instead of fifteen.
+template <typename R>
+struct Ta9 {
+    R v[9];
+};
+
+template <typename R>
+LGF_DEV void set_zero(Ta9<R>& t) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t.v[i] = R(0);
+}
+
+template <typename R>
+LGF_DEV void scale(Ta9<R>& t, R w) {
+#pragma unroll
+    for (int i = 0; i < 9; ++i) t.v[i] *= w;
+}
+
+#define LGF_TA9_TERMS(AIDX, BIDX, CONJA, CONJB, SX)      \
+    _Pragma("unroll") for (int k = 0; k < 3; ++k) {       \
+        const R ar = a.v[AIDX], ai = CONJA a.v[AIDX + 1]; \
+        const R br = b.v[BIDX], bi = CONJB b.v[BIDX + 1]; \
+        x SX ar * br;                                     \
+        x SX - ai * bi;                                   \
+        y += ar * bi;                                     \
+        y += ai * br;                                     \
+    }
+#define LGF_TA9_BODY(AIDX, BIDX, CONJA, CONJB)                                     \
+    _Pragma("unroll") for (int d = 0; d < 3; ++d) {                                \
+        const int r = d, c = d;                                                    \
+        R y = o.v[d];                                                              \
+        _Pragma("unroll") for (int k = 0; k < 3; ++k) {                            \
+            y += a.v[AIDX] * (CONJB b.v[BIDX + 1]);                                \
+            y += (CONJA a.v[AIDX + 1]) * b.v[BIDX];                                \
+        }                                                                          \
+        o.v[d] = y;                                                                \
+    }                                                                              \
+    _Pragma("unroll") for (int p = 0; p < 3; ++p) {                                \
+        const int r0 = p == 2 ? 1 : 0, c0 = p == 0 ? 1 : 2;                        \
+        R x = o.v[3 + 2 * p], y = o.v[4 + 2 * p];                                  \
+        {                                                                          \
+            const int r = r0, c = c0;                                              \
+            LGF_TA9_TERMS(AIDX, BIDX, CONJA, CONJB, +=)                            \
+        }                                                                          \
+        {                                                                          \
+            const int r = c0, c = r0;                                              \
+            LGF_TA9_TERMS(AIDX, BIDX, CONJA, CONJB, -=)                            \
+        }                                                                          \
+        o.v[3 + 2 * p] = x;                                                        \
+        o.v[4 + 2 * p] = y;                                                        \
+    }
+
+// o += a b, a b^dag, a^dag b in the folded form
+template <typename R>
+LGF_DEV void mul_nn_acc_ta(Ta9<R>& o, const M3<R>& a, const M3<R>& b) {
+    LGF_TA9_BODY(2 * (3 * r + k), 2 * (3 * k + c), , )
+}
+template <typename R>
+LGF_DEV void mul_nd_acc_ta(Ta9<R>& o, const M3<R>& a, const M3<R>& b) {
+    LGF_TA9_BODY(2 * (3 * r + k), 2 * (3 * c + k), , -)
+}
+template <typename R>
+LGF_DEV void mul_dn_acc_ta(Ta9<R>& o, const M3<R>& a, const M3<R>& b) {
+    LGF_TA9_BODY(2 * (3 * k + r), 2 * (3 * k + c), -, )
+}
+#undef LGF_TA9_BODY
+#undef LGF_TA9_TERMS
+
 // Products of two SU(3) matrices whose third row is rebuilt as the
 // conjugated cross product of the first two (exact for SU(3)): 96 FMAs
 // instead of 108.
