@@ -566,7 +566,30 @@ struct Lines {
     const C* L3;  // lines of length 3
     int64_t G;    // line stride
     int lbase;    // line index of site 0 of this field
+    const C* xl;  // LDS copy of the 12 lines that start at the workgroup's sites, or null
+    int lane;
 };
+
+// With FP64 links (the parity path) the twelve lines that start at a site
+// (three lengths, four directions) are loaded once per workgroup into LDS,
+// where the six plane waves of the site read them: a quarter of the sweep's
+// line loads, all left factors (the site's own lines in the (+nu,+mu) leaves
+// and the first factor of the side leaves).  That kernel is bound by its
+// line loads (VALU about a third busy) and holds one workgroup per CU at
+// its register budget, so the 110 KB of LDS cost no occupancy; the cache
+// shares its LDS with the per-plane field strengths, which are written
+// after every wave is done with it.
+template <typename S>
+constexpr bool kCloverXLines = sizeof(S) == 8;
+
+template <int LEN, typename S, typename C>
+LGF_DEV M3<C> xline(const Lines<S, C>& L, int dir) {
+    M3<C> m;
+    const C* q = L.xl + (size_t)(4 * (LEN - 1) + dir) * 18 * kTileSites + L.lane;
+#pragma unroll
+    for (int c = 0; c < 18; ++c) m.v[c] = q[c * kTileSites];
+    return m;
+}
 
 // Products of lines.  With FP32 links (the mixed-precision path, whose links
 // are projected onto SU(3) every flow step) they are SU(3) products: rows 0
@@ -635,12 +658,6 @@ LGF_DEV M3<C> clover_nd(const M3<C>& a, const M3<C>& b) {
     else return mul_nd(a, b);
 }
 
-// In FP64 arithmetic (kCloverChain) a leaf's products are chained: the
-// lines of each product are loaded after the previous product is formed, so
-// one product's operands are live at a time.  The FP32 kernel keeps a
-// leaf's loads together (more in flight) and fits its budget as it is.
-template <typename C>
-constexpr bool kCloverChain = sizeof(C) == 8;
 
 // diagonal quadrants: it = 0 is (+nu,+mu), it = 1 is (-nu,-mu)
 template <int M, int N, typename S, typename C>
@@ -654,11 +671,13 @@ LGF_DEV void diag_leaf(Ta9<C>& acc, C w, const Lines<S, C>& L, int s, const Plan
     const int mb = o.m[3 - M], nb = o.n[3 - N], mf = o.m[3 + M], nf = o.n[3 + N];
     const int sc = it ? s + nb + mb : s;
     const int scn = it ? s + mb : s + nf;  // c + n nu
-    M3<C> P = clover_nn<S, C>(line<N, 2>(L, nu, sc), line<M, 3>(L, mu, scn));
-    const int s2 = kCloverChain<C> ? after_value(s, P.v[17]) : s;
-    const int sc2 = it ? s2 + nb + mb : s2;
-    const int scm = it ? s2 + nb : s2 + mf;  // c + m mu
-    const M3<C> R = clover_nn<S, C>(line<M, 2>(L, mu, sc2), line<N, 3>(L, nu, scm));
+    M3<C> P;
+    if (kCloverXLines<S> && it == 0) P = clover_nn<S, C>(xline<N>(L, nu), line<M, 3>(L, mu, scn));
+    else P = clover_nn<S, C>(line<N, 2>(L, nu, sc), line<M, 3>(L, mu, scn));
+    const int scm = it ? s + nb : s + mf;  // c + m mu
+    M3<C> R;
+    if (kCloverXLines<S> && it == 0) R = clover_nn<S, C>(xline<M>(L, mu), line<N, 3>(L, nu, scm));
+    else R = clover_nn<S, C>(line<M, 2>(L, mu, sc), line<N, 3>(L, nu, scm));
     if (w != C(1)) scale(P, w);
     // the clover sums enter only through their traceless anti-Hermitian part
     if (it == 0) mul_nd_acc_ta(acc, P, R);
@@ -680,11 +699,9 @@ LGF_DEV void side_leaf(Ta9<C>& acc, C w, const Lines<S, C>& L, int s, const Plan
     const int zb = it ? m2 : n2;  // z = x - l2 beta
     const int za = it ? n1 : m1;  // + l1 alpha
     const M3<C> H = clover_nn<S, C>(line<L1, 2>(L, a, s + zb), line<L2, 3>(L, b, s + zb + za));
-    const int s2 = kCloverChain<C> ? after_value(s, H.v[17]) : s;
-    M3<C> T = clover_nd<S, C>(line<L1, 2>(L, a, s2), H);
+    M3<C> T = clover_nd<S, C>(kCloverXLines<S> ? xline<L1>(L, a) : line<L1, 2>(L, a, s), H);
     scale(T, it ? -w : w);
-    const int s3 = kCloverChain<C> ? after_value(s2, T.v[17]) : s2;
-    mul_nn_acc_ta(acc, T, line<L2, 3>(L, b, s3 + zb));
+    mul_nn_acc_ta(acc, T, line<L2, 3>(L, b, s + zb));
 }
 
 // F = 1/2 (-i) [ (C - C^dag) - 1/3 Tr(C - C^dag) ] (LGF/Field/SU3_field.cpp:575-579)
@@ -721,7 +738,8 @@ template <typename C, typename T>
 LGF_DEV void herm_pack(const M3<C>& F, T* h, int step);
 
 template <typename S, typename C, typename T>
-LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, const double* rcp, int mu, int nu, M3<C>& Fimp, T* f11) {
+LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, const double* rcp, int mu, int nu, M3<C>& Fimp, T* f11,
+                  int f11_step) {
     PlaneOff o;
 #pragma unroll
     for (int k = -3; k <= 3; ++k) {
@@ -741,7 +759,7 @@ LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, const double* rcp, int mu,
     {
         const M3<C> F11 = project_F(acc);
         e = retr_mul(F11, F11);
-        herm_pack(F11, f11, kTileSites);
+        herm_pack(F11, f11, f11_step);
     }
     // Leaves of one weight are added unscaled and the sum is rescaled between
     // weights: acc = k1 avg11 / w22, + S22, * w22/w12, + S12, * w12/w13,
@@ -850,36 +868,80 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     int64_t G, double* __restrict__ partials) {
     // F values are computed in C; keeping them as C in LDS loses nothing
     using T = typename std::conditional<LGF_CLOVER_LDS_REAL != 0, C, double>::type;
-    __shared__ T sF[6][9][kTileSites];
-    __shared__ T sF11[6][9][kTileSites];
-    __shared__ double sE[6][kTileSites];
-    __shared__ double sEi[6][kTileSites];
+    constexpr bool XL = kCloverXLines<S>;
+    struct Out {
+        T F[6][9][kTileSites];
+        T F11[6][9][kTileSites];
+        double E[6][kTileSites];
+        double Ei[6][kTileSites];
+    };
+    union Shared {
+        Out o;
+        C xl[XL ? 12 : 1][18][kTileSites];
+    };
+    __shared__ Shared sh;
+    T (*sF)[9][kTileSites] = sh.o.F;
+    T (*sF11)[9][kTileSites] = sh.o.F11;
+    double (*sE)[kTileSites] = sh.o.E;
+    double (*sEi)[kTileSites] = sh.o.Ei;
     const int tix = xcd_remap(blockIdx.x, gridDim.x);
     const int2 tm = tiles[tix];
     const FieldDesc& d = fds[tm.x];
     const int s = tm.y * kTileSites + threadIdx.x;
     const int p = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const bool live = s < d.vol;
+    Lines<S, C> L;
+    L.U = (const S*)d.links[ping ^ d.flip];
+    L.st = d.stride;
+    L.L2 = lines;
+    L.L3 = lines + 4 * 18 * G;
+    L.G = G;
+    L.lbase = (d.tile0 - t0) * kTileSites;
+    L.xl = XL ? &sh.xl[0][0][0] : nullptr;
+    L.lane = threadIdx.x;
+    if constexpr (XL) {
+        // wave p stages lines 2p and 2p+1 (line k: length k/4 + 1, direction k%4)
+        if (live) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int k = 2 * p + h, dir = k & 3;
+                const M3<C> m = k < 4 ? load_link<S, C>(L.U, L.st, dir, s)
+                                      : load_link<C, C>(k < 8 ? L.L2 : L.L3, G, dir, L.lbase + s);
+#pragma unroll
+                for (int c = 0; c < 18; ++c) sh.xl[k][c][threadIdx.x] = m.v[c];
+            }
+        }
+        __syncthreads();
+    }
     double e = 0.0, ei = 0.0;
+    T f11r[9], fir[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) f11r[i] = fir[i] = T(0);
     if (live) {
-        Lines<S, C> L;
-        L.U = (const S*)d.links[ping ^ d.flip];
-        L.st = d.stride;
-        L.L2 = lines;
-        L.L3 = lines + 4 * 18 * G;
-        L.G = G;
-        L.lbase = (d.tile0 - t0) * kTileSites;
         const Geo g = make_geo(d, s);
         // plane p: (3,2),(0,1),(3,1),(2,0),(3,0),(1,2)
         const int mu = (p & 1) ? (p == 1 ? 0 : (p == 3 ? 2 : 1)) : 3;
         const int nu = (p & 1) ? (p == 1 ? 1 : (p == 3 ? 0 : 2)) : (p == 0 ? 2 : (p == 2 ? 1 : 0));
         M3<C> Fi;
-        e = (double)plane_F<S, C, T>(L, g, d.rcp, mu, nu, Fi, &sF11[p][0][threadIdx.x]);
+        if constexpr (XL) {
+            e = (double)plane_F<S, C, T>(L, g, d.rcp, mu, nu, Fi, f11r, 1);
+            herm_pack(Fi, fir, 1);
+        } else {
+            e = (double)plane_F<S, C, T>(L, g, d.rcp, mu, nu, Fi, &sF11[p][0][threadIdx.x], kTileSites);
+            herm_pack(Fi, &sF[p][0][threadIdx.x], kTileSites);
+        }
         ei = (double)retr_mul(Fi, Fi);
-        herm_pack(Fi, &sF[p][0][threadIdx.x], kTileSites);
-    } else {
+    } else if constexpr (!XL) {
 #pragma unroll
         for (int i = 0; i < 9; ++i) sF[p][i][threadIdx.x] = sF11[p][i][threadIdx.x] = T(0);
+    }
+    if constexpr (XL) {
+        __syncthreads();  // every wave is done with the line cache
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            sF[p][i][threadIdx.x] = fir[i];
+            sF11[p][i][threadIdx.x] = f11r[i];
+        }
     }
     sE[p][threadIdx.x] = e;
     sEi[p][threadIdx.x] = ei;
