@@ -506,21 +506,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 // clovers, E, Q, q(x)
 // ---------------------------------------------------------------------------
 // Compile-time knobs of the clover kernel:
-//   LGF_CLOVER_FENCE    1: a leaf's loads are not hoisted into the previous leaf
-//   LGF_CLOVER_WPE      minimum waves per SIMD of the register budget
+//   LGF_CLOVER_WPE      minimum waves per SIMD of the register budget (FP32)
+//   LGF_CLOVER_WPE64    the same for FP64 arithmetic on FP32 links (the
+//                       mixed-precision path's final sweep)
+//   LGF_CLOVER_WPE64_PARITY  the same for FP64 links
 //   LGF_CLOVER_LDS_REAL 1: per-plane field strengths meet in LDS in the
 //                       arithmetic precision (float for FP32 clovers), which
 //                       halves LDS use and lets more workgroups share a CU
-#ifndef LGF_CLOVER_FENCE
-#define LGF_CLOVER_FENCE 0
-#endif
+//   LGF_CLOVER_ORDERED  1: the addresses (and so the loads) of a pair of
+//                       leaves depend on the previous pair's result, so few
+//                       lines are in flight at a time (fewer registers, more
+//                       waves)
+//   LGF_CLOVER_ORDERED_INNER 1: also between the two leaves of a pair
+// Measured on 24^4 (three sweep precisions): ORDERED 0 spills and doubles
+// the sweep times; ORDERED_INNER 1 costs the FP64-arithmetic sweep 4 %;
+// three waves per SIMD for that sweep 30 %.
 #ifndef LGF_CLOVER_WPE
 #define LGF_CLOVER_WPE 4
 #endif
-//   LGF_CLOVER_WPE64    the same for FP64 arithmetic on FP32 links (the
-//                       mixed-precision path's final sweep) and on FP64 links
 #ifndef LGF_CLOVER_WPE64
-#define LGF_CLOVER_WPE64 3
+#define LGF_CLOVER_WPE64 2
 #endif
 #ifndef LGF_CLOVER_WPE64_PARITY
 #define LGF_CLOVER_WPE64_PARITY 2
@@ -528,11 +533,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 #ifndef LGF_CLOVER_LDS_REAL
 #define LGF_CLOVER_LDS_REAL 1
 #endif
-//   LGF_CLOVER_ORDERED  1: a leaf's addresses (and so its loads) depend on the
-//                       previous leaf's result, so one leaf's lines are in
-//                       flight at a time (fewer registers, more waves)
 #ifndef LGF_CLOVER_ORDERED
 #define LGF_CLOVER_ORDERED 1
+#endif
+#ifndef LGF_CLOVER_ORDERED_INNER
+#define LGF_CLOVER_ORDERED_INNER 0
 #endif
 
 // Line products.  L^d_l(y) = U_d(y) U_d(y+d) ... U_d(y+(l-1)d), the product
@@ -540,8 +545,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 // site and direction of the fields being measured (precision C, the vector
 // plane layout of the links, line index lbase + site); L_1 is the link.
 //
-// With them every clover leaf of extent m along mu and n along nu is three
-// matrix products.  The reference's four quadrant leaves of the (mu,nu)
+// With them every clover leaf of extent m along mu and n along nu is a
+// product of four lines.  The reference's four quadrant leaves of the (mu,nu)
 // plane (LGF/Field/SU3_field.cpp:533-584 for 1x1, 598-792 for 2x2, 1x2 and
 // 1x3), each a closed loop from x, are:
 //   quadrant (+nu,+mu): W = P(x) R(x)^dag
@@ -553,11 +558,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 //     H' = L^nu_n(z') L^mu_m(z' + n nu)
 // (e.g. the first 1x1 leaf is U_nu(x) U_mu(x+nu) U_nu(x+mu)^dag U_mu(x)^dag,
 // LGF/Field/SU3_field.cpp:550-560).  The field strength depends on a clover
-// sum only through C - C^dag, to which W and -W^dag contribute alike, so the
-// last quadrant enters as -W^dag; the two "side" quadrants then share one
-// code path with mu and nu exchanged, and the two "diagonal" quadrants share
-// one up to the last product.  Leaf shapes come in both orientations with
-// equal weights, so a shape loop over (l1, l2) covers both.
+// sum only through C - C^dag, to which W and -W^dag contribute alike.  Leaf
+// shapes come in both orientations (m x n and n x m) with equal weights.
 template <typename S, typename C>
 struct Lines {
     const S* U;   // current links: lines of length 1
@@ -572,15 +574,18 @@ struct Lines {
 
 // With FP64 links (the parity path) the twelve lines that start at a site
 // (three lengths, four directions) are loaded once per workgroup into LDS,
-// where the six plane waves of the site read them: a quarter of the sweep's
-// line loads, all left factors (the site's own lines in the (+nu,+mu) leaves
-// and the first factor of the side leaves).  That kernel is bound by its
-// line loads (VALU about a third busy) and holds one workgroup per CU at
-// its register budget, so the 110 KB of LDS cost no occupancy; the cache
-// shares its LDS with the per-plane field strengths, which are written
-// after every wave is done with it.
+// where the six plane waves of the site read them: a fifth of the sweep's
+// line loads (a_n, R1 and the first factor of T3 in plane_F_factored).  That
+// kernel is bound by its line loads (VALU about a third busy) and holds one
+// workgroup per CU at its register budget, so the 110 KB of LDS cost no
+// occupancy; the cache shares its LDS with the per-plane field strengths,
+// which are written after every wave is done with it.  Without it the
+// sweep is 9 % slower (24^4).
+#ifndef LGF_CLOVER_XLINES
+#define LGF_CLOVER_XLINES 1
+#endif
 template <typename S>
-constexpr bool kCloverXLines = sizeof(S) == 8;
+constexpr bool kCloverXLines = LGF_CLOVER_XLINES && sizeof(S) == 8;
 
 template <int LEN, typename S, typename C>
 LGF_DEV M3<C> xline(const Lines<S, C>& L, int dir) {
@@ -593,31 +598,37 @@ LGF_DEV M3<C> xline(const Lines<S, C>& L, int dir) {
 
 // Products of lines.  With FP32 links (the mixed-precision path, whose links
 // are projected onto SU(3) every flow step) they are SU(3) products: rows 0
-// and 1 and their conjugated cross product.  A line that is only ever a left
-// factor is then read as rows 0 and 1 (ROWS = 2), a right factor whole
-// (ROWS = 3).  In the FP64-arithmetic sweep over FP32 links (kCloverRows) a
-// right factor too is read as two rows and its third rebuilt: that sweep is
-// bound by the bytes its lines move through the vector memory path (VALU
-// 30-40 % busy), and a rebuilt row costs 24 FMAs against 48 of 144 bytes.
-// The FP32 sweep is bound by its arithmetic and reads right factors whole.
-// With FP64 links (the parity path) the products are full, as the
-// reference's: its links are not projected, and rebuilding rows would move
-// the result by the links' drift from SU(3).
+// and 1 and their conjugated cross product.  Every line is then read as rows
+// 0 and 1 (kCloverRows, and k_lines stores only those), and where a product
+// needs more of it (a right factor, or columns 0, 1 of a left factor) its
+// third row is rebuilt: both sweeps over FP32 links are bound by the bytes
+// their loads return through the vector memory path (24^4, FP32 sweep:
+// texture data unit 88-91 % busy, VALU 54 %), and a rebuilt row costs 24
+// FMAs against 24 of 72 bytes (FP32) or 48 of 144 (FP64).  With FP64 links
+// (the parity path) the products are full, as the reference's: its links are
+// not projected, and rebuilding rows would move the result by the links'
+// drift from SU(3).
 #ifndef LGF_CLOVER_ROWS2
 #define LGF_CLOVER_ROWS2 1
 #endif
 template <typename S>
 constexpr bool kCloverSu3 = sizeof(S) == 4;
 template <typename S, typename C>
-constexpr bool kCloverRows = LGF_CLOVER_ROWS2 && kCloverSu3<S> && sizeof(C) == 8;
+constexpr bool kCloverRows = LGF_CLOVER_ROWS2 && kCloverSu3<S>;
 
+// ROWS = 2: rows 0, 1; 3: whole; 4: columns 0, 1 (a left factor of
+// mul_dd_r01 or a right factor of mul_nn_c01), which for whole FP32 lines
+// are the four float4 planes.
 template <int LEN, int ROWS, typename S, typename C>
 LGF_DEV M3<C> line(const Lines<S, C>& L, int dir, int site) {
     if constexpr (kCloverSu3<S> && (ROWS == 2 || kCloverRows<S, C>)) {
         M3<C> m = LEN == 1 ? load_link_rows01<S, C>(L.U, L.st, dir, site)
                            : load_link_rows01<C, C>(LEN == 2 ? L.L2 : L.L3, L.G, dir, L.lbase + site);
-        if constexpr (ROWS == 3) su3_third_row(m);
+        if constexpr (ROWS != 2) su3_third_row(m);
         return m;
+    } else if constexpr (kCloverSu3<S> && ROWS == 4) {
+        return LEN == 1 ? load_link_cols01<C>(L.U, L.st, dir, site)
+                        : load_link_cols01<C>(LEN == 2 ? L.L2 : L.L3, L.G, dir, L.lbase + site);
     } else if constexpr (LEN == 1) {
         return load_link<S, C>(L.U, L.st, dir, site);
     } else {
@@ -641,67 +652,100 @@ struct PlaneOff {
     int m[7], n[7];
 };
 
-template <typename C>
-LGF_DEV void scale(M3<C>& a, C w) {
-#pragma unroll
-    for (int i = 0; i < 18; ++i) a.v[i] *= w;
+// Factored clover.  Every leaf of the plane is an outer line, a middle
+// product and another outer line, and the four quadrants pair up on a
+// shared outer factor, so the leaves of one n (extent along nu) are summed
+// as 3x3 matrices before one projected product with that factor:
+//   (+nu,+mu):  W = a_n Mid1 R1^dag,   a_n = L^nu_n(x), R1 = L^mu_m(x),
+//               Mid1 = L^mu_m(x+n nu) L^nu_n(x+m mu)^dag
+//   (-mu,+nu):  W^dag = a_n Mid4 R4,   R4 = L^mu_m(x-m mu),
+//               Mid4 = L^mu_m(x-m mu+n nu)^dag L^nu_n(x-m mu)^dag
+//   (+mu,-nu):  W = T3 g_n,            g_n = L^nu_n(x-n nu), T3 = L^mu_m(x) H3^dag,
+//               H3 = L^mu_m(x-n nu) L^nu_n(x-n nu+m mu)
+//   (-nu,-mu):  W^dag = T2 g_n,        T2 = K2^dag L^mu_m(c), c = x-n nu-m mu,
+//               K2 = L^nu_n(c) L^mu_m(x-m mu)
+// (the quadrant leaves above regrouped; TA(W^dag) = -TA(W)).
+// So TA(sum of leaves) = TA(a_n Y_n) + TA(Z_n g_n) summed over n, with
+//   Y_n = sum_m w_mn (Mid1 R1^dag - Mid4 R4),  Z_n = sum_m w_mn (T3 - T2).
+// Per leaf that is two products instead of three, the third product shared
+// by the leaves of one n.  With FP32 links (SU(3) products) the middle
+// product is needed only in the rows or columns that the second product
+// reads, 72 FMAs.
+
+// a whole line that starts at the workgroup's own site x
+template <int LEN, typename S, typename C>
+LGF_DEV M3<C> own_line(const Lines<S, C>& L, int dir, int s) {
+    if constexpr (kCloverXLines<S>) return xline<LEN>(L, dir);
+    else return line<LEN, 3>(L, dir, s);
 }
 
-template <typename S, typename C>
-LGF_DEV M3<C> clover_nn(const M3<C>& a, const M3<C>& b) {
-    if constexpr (kCloverSu3<S>) return mul_nn_su3(a, b);
-    else return mul_nn(a, b);
-}
-template <typename S, typename C>
-LGF_DEV M3<C> clover_nd(const M3<C>& a, const M3<C>& b) {
-    if constexpr (kCloverSu3<S>) return mul_nd_su3(a, b);
-    else return mul_nd(a, b);
-}
-
-
-// diagonal quadrants: it = 0 is (+nu,+mu), it = 1 is (-nu,-mu)
 template <int M, int N, typename S, typename C>
-LGF_DEV void diag_leaf(Ta9<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
-#if LGF_CLOVER_FENCE
-    asm volatile("" ::: "memory");  // keep the next leaf's loads from being hoisted into this one
-#endif
+LGF_DEV void leaves_a(M3<C>& Y, C w, bool first, C dep, const Lines<S, C>& L, int s, const PlaneOff& o, int mu,
+                      int nu) {
 #if LGF_CLOVER_ORDERED
-    asm volatile("" : "+v"(s) : "v"(acc.v[8]));
+    asm volatile("" : "+v"(s) : "v"(dep));
 #endif
-    const int mb = o.m[3 - M], nb = o.n[3 - N], mf = o.m[3 + M], nf = o.n[3 + N];
-    const int sc = it ? s + nb + mb : s;
-    const int scn = it ? s + mb : s + nf;  // c + n nu
-    M3<C> P;
-    if (kCloverXLines<S> && it == 0) P = clover_nn<S, C>(xline<N>(L, nu), line<M, 3>(L, mu, scn));
-    else P = clover_nn<S, C>(line<N, 2>(L, nu, sc), line<M, 3>(L, mu, scn));
-    const int scm = it ? s + nb : s + mf;  // c + m mu
-    M3<C> R;
-    if (kCloverXLines<S> && it == 0) R = clover_nn<S, C>(xline<M>(L, mu), line<N, 3>(L, nu, scm));
-    else R = clover_nn<S, C>(line<M, 2>(L, mu, sc), line<N, 3>(L, nu, scm));
-    if (w != C(1)) scale(P, w);
-    // the clover sums enter only through their traceless anti-Hermitian part
-    if (it == 0) mul_nd_acc_ta(acc, P, R);
-    else mul_dn_acc_ta(acc, R, P);
+    const int sm = s + o.m[3 - M];  // x - m mu
+    M3<C> p1, p4;
+    if constexpr (kCloverSu3<S>) {
+        const M3<C> mid1 = mul_nd_r01(line<M, 2>(L, mu, s + o.n[3 + N]), line<N, 3>(L, nu, s + o.m[3 + M]));
+        p1 = mul_nd_su3(mid1, line<M, 3>(L, mu, s));
+    } else {
+        const M3<C> mid1 = mul_nd(line<M, 3>(L, mu, s + o.n[3 + N]), line<N, 3>(L, nu, s + o.m[3 + M]));
+        p1 = mul_nd(mid1, own_line<M>(L, mu, s));
+    }
+    if (first) set_scaled(Y, w, p1);
+    else axpy(Y, w, p1);
+#if LGF_CLOVER_ORDERED && LGF_CLOVER_ORDERED_INNER
+    int s4 = sm;
+    asm volatile("" : "+v"(s4) : "v"(Y.v[17]));
+#else
+    const int s4 = sm;
+#endif
+    if constexpr (kCloverSu3<S>) {
+        const M3<C> mid4 = mul_dd_r01(line<M, 4>(L, mu, s4 + o.n[3 + N]), line<N, 3>(L, nu, s4));
+        p4 = mul_nn_su3(mid4, line<M, 3>(L, mu, s4));
+    } else {
+        const M3<C> h4 = mul_nn(line<N, 3>(L, nu, s4), line<M, 3>(L, mu, s4 + o.n[3 + N]));
+        p4 = mul_dn(h4, line<M, 3>(L, mu, s4));
+    }
+    axpy(Y, -w, p4);
 }
 
-// side quadrants: it = 0 is (+mu,-nu) with (alpha, beta) = (mu, nu); it = 1
-// is (-mu,+nu) with (alpha, beta) = (nu, mu), entering as -W^dag
-template <int L1, int L2, typename S, typename C>
-LGF_DEV void side_leaf(Ta9<C>& acc, C w, const Lines<S, C>& L, int s, const PlaneOff& o, int mu, int nu, int it) {
-#if LGF_CLOVER_FENCE
-    asm volatile("" ::: "memory");
-#endif
+template <int M, int N, typename S, typename C>
+LGF_DEV void leaves_b(M3<C>& Z, C w, bool first, C dep, const Lines<S, C>& L, int s, const PlaneOff& o, int mu,
+                      int nu) {
 #if LGF_CLOVER_ORDERED
-    asm volatile("" : "+v"(s) : "v"(acc.v[8]));
+    asm volatile("" : "+v"(s) : "v"(dep));
 #endif
-    const int a = it ? nu : mu, b = it ? mu : nu;
-    const int m2 = o.m[3 - L2], n2 = o.n[3 - L2], m1 = o.m[3 + L1], n1 = o.n[3 + L1];
-    const int zb = it ? m2 : n2;  // z = x - l2 beta
-    const int za = it ? n1 : m1;  // + l1 alpha
-    const M3<C> H = clover_nn<S, C>(line<L1, 2>(L, a, s + zb), line<L2, 3>(L, b, s + zb + za));
-    M3<C> T = clover_nd<S, C>(kCloverXLines<S> ? xline<L1>(L, a) : line<L1, 2>(L, a, s), H);
-    scale(T, it ? -w : w);
-    mul_nn_acc_ta(acc, T, line<L2, 3>(L, b, s + zb));
+    const int sn = s + o.n[3 - N];  // x - n nu
+    if constexpr (kCloverSu3<S>) {
+        const M3<C> h3 = mul_nn_r01(line<M, 2>(L, mu, sn), line<N, 3>(L, nu, sn + o.m[3 + M]));
+        const M3<C> t3d = mul_nd_su3(h3, line<M, 3>(L, mu, s));  // T3^dag
+        if (first) set_scaled_dag(Z, w, t3d);
+        else axpy_dag(Z, w, t3d);
+    } else {
+        const M3<C> h3 = mul_nn(line<M, 3>(L, mu, sn), line<N, 3>(L, nu, sn + o.m[3 + M]));
+        const M3<C> t3 = mul_nd(own_line<M>(L, mu, s), h3);
+        if (first) set_scaled(Z, w, t3);
+        else axpy(Z, w, t3);
+    }
+#if LGF_CLOVER_ORDERED && LGF_CLOVER_ORDERED_INNER
+    int sc = sn + o.m[3 - M];  // c
+    asm volatile("" : "+v"(sc) : "v"(Z.v[17]));
+#else
+    const int sc = sn + o.m[3 - M];
+#endif
+    const int smm = s + o.m[3 - M];  // x - m mu
+    M3<C> t2;
+    if constexpr (kCloverSu3<S>) {
+        const M3<C> k2 = mul_nn_c01(line<N, 3>(L, nu, sc), line<M, 4>(L, mu, smm));
+        t2 = mul_dn_su3(k2, line<M, 3>(L, mu, sc));
+    } else {
+        const M3<C> k2 = mul_nn(line<N, 3>(L, nu, sc), line<M, 3>(L, mu, smm));
+        t2 = mul_dn(k2, line<M, 3>(L, mu, sc));
+    }
+    axpy(Z, -w, t2);
 }
 
 // F = 1/2 (-i) [ (C - C^dag) - 1/3 Tr(C - C^dag) ] (LGF/Field/SU3_field.cpp:575-579)
@@ -738,6 +782,53 @@ template <typename C, typename T>
 LGF_DEV void herm_pack(const M3<C>& F, T* h, int step);
 
 template <typename S, typename C, typename T>
+LGF_DEV C plane_F_factored(const Lines<S, C>& L, const PlaneOff& o, int s, int mu, int nu, M3<C>& Fimp, T* f11,
+                           int f11_step) {
+    const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
+    // per-leaf weights of the improved sum: k1/4 of the four square 1x1
+    // leaves, k2/4 of the 2x2, k3/8 and k4/8 of the eight 1x2 and 1x3
+    const C w22 = C(0.25) * k2, w12 = C(0.125) * k3, w13 = C(0.125) * k4;
+    Ta9<C> a11, acc;  // the 1x1 sum (plain F) and the weighted sum of the larger leaves
+    set_zero(a11);
+    set_zero(acc);
+    M3<C> Y;
+    // quadrants (+nu,+mu), (-mu,+nu): TA(a_n Y_n)
+    leaves_a<1, 1>(Y, C(1), true, acc.v[8], L, s, o, mu, nu);
+    mul_nn_acc_ta(a11, own_line<1>(L, nu, s), Y);
+    leaves_a<2, 1>(Y, w12, true, a11.v[8], L, s, o, mu, nu);
+    leaves_a<3, 1>(Y, w13, false, Y.v[17], L, s, o, mu, nu);
+    mul_nn_acc_ta(acc, own_line<1>(L, nu, s), Y);
+    leaves_a<1, 2>(Y, w12, true, acc.v[8], L, s, o, mu, nu);
+    leaves_a<2, 2>(Y, w22, false, Y.v[17], L, s, o, mu, nu);
+    mul_nn_acc_ta(acc, own_line<2>(L, nu, s), Y);
+    leaves_a<1, 3>(Y, w13, true, acc.v[8], L, s, o, mu, nu);
+    mul_nn_acc_ta(acc, own_line<3>(L, nu, s), Y);
+    // quadrants (+mu,-nu), (-nu,-mu): TA(Z_n g_n)
+    leaves_b<1, 1>(Y, C(1), true, acc.v[8], L, s, o, mu, nu);
+    mul_nn_acc_ta(a11, Y, line<1, 3>(L, nu, s + o.n[2]));
+    leaves_b<2, 1>(Y, w12, true, a11.v[8], L, s, o, mu, nu);
+    leaves_b<3, 1>(Y, w13, false, Y.v[17], L, s, o, mu, nu);
+    mul_nn_acc_ta(acc, Y, line<1, 3>(L, nu, s + o.n[2]));
+    leaves_b<1, 2>(Y, w12, true, acc.v[8], L, s, o, mu, nu);
+    leaves_b<2, 2>(Y, w22, false, Y.v[17], L, s, o, mu, nu);
+    mul_nn_acc_ta(acc, Y, line<2, 3>(L, nu, s + o.n[1]));
+    leaves_b<1, 3>(Y, w13, true, acc.v[8], L, s, o, mu, nu);
+    mul_nn_acc_ta(acc, Y, line<3, 3>(L, nu, s + o.n[0]));
+    C e;
+    {
+        Ta9<C> avg = a11;
+        scale(avg, C(0.25));
+        const M3<C> F11 = project_F(avg);
+        e = retr_mul(F11, F11);
+        herm_pack(F11, f11, f11_step);
+    }
+#pragma unroll
+    for (int i = 0; i < 9; ++i) acc.v[i] += (C(0.25) * k1) * a11.v[i];
+    Fimp = project_F(acc);
+    return e;
+}
+
+template <typename S, typename C, typename T>
 LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, const double* rcp, int mu, int nu, M3<C>& Fimp, T* f11,
                   int f11_step) {
     PlaneOff o;
@@ -746,51 +837,7 @@ LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, const double* rcp, int mu,
         o.m[3 + k] = k == 0 ? 0 : wrap_offset(g, mu, k, rcp);
         o.n[3 + k] = k == 0 ? 0 : wrap_offset(g, nu, k, rcp);
     }
-    const int s = g.s;
-    const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
-    Ta9<C> acc;
-    set_zero(acc);
-#pragma unroll 1
-    for (int it = 0; it < 2; ++it) diag_leaf<1, 1>(acc, C(1), L, s, o, mu, nu, it);
-#pragma unroll 1
-    for (int it = 0; it < 2; ++it) side_leaf<1, 1>(acc, C(1), L, s, o, mu, nu, it);
-    scale(acc, C(0.25));
-    C e;
-    {
-        const M3<C> F11 = project_F(acc);
-        e = retr_mul(F11, F11);
-        herm_pack(F11, f11, f11_step);
-    }
-    // Leaves of one weight are added unscaled and the sum is rescaled between
-    // weights: acc = k1 avg11 / w22, + S22, * w22/w12, + S12, * w12/w13,
-    // + S13, * w13, with w22 = k2/4, w12 = k3/8, w13 = k4/8 the per-leaf
-    // weights of the 2x2, 1x2 and 1x3 clover averages.
-    const C w22 = C(0.25) * k2, w12 = C(0.125) * k3, w13 = C(0.125) * k4;
-    scale(acc, k1 / w22);
-#pragma unroll 1
-    for (int it = 0; it < 2; ++it) {
-        diag_leaf<2, 2>(acc, C(1), L, s, o, mu, nu, it);
-        side_leaf<2, 2>(acc, C(1), L, s, o, mu, nu, it);
-    }
-    scale(acc, w22 / w12);
-#pragma unroll 1
-    for (int it = 0; it < 2; ++it) {
-        diag_leaf<1, 2>(acc, C(1), L, s, o, mu, nu, it);
-        diag_leaf<2, 1>(acc, C(1), L, s, o, mu, nu, it);
-        side_leaf<1, 2>(acc, C(1), L, s, o, mu, nu, it);
-        side_leaf<2, 1>(acc, C(1), L, s, o, mu, nu, it);
-    }
-    scale(acc, w12 / w13);
-#pragma unroll 1
-    for (int it = 0; it < 2; ++it) {
-        diag_leaf<1, 3>(acc, C(1), L, s, o, mu, nu, it);
-        diag_leaf<3, 1>(acc, C(1), L, s, o, mu, nu, it);
-        side_leaf<1, 3>(acc, C(1), L, s, o, mu, nu, it);
-        side_leaf<3, 1>(acc, C(1), L, s, o, mu, nu, it);
-    }
-    scale(acc, w13);
-    Fimp = project_F(acc);
-    return e;
+    return plane_F_factored<S, C, T>(L, o, g.s, mu, nu, Fimp, f11, f11_step);
 }
 
 // Re Tr(A B) for Hermitian A, B given as 9 reals each:

@@ -191,6 +191,24 @@ LGF_DEV M3<C> load_link_rows01(const S* __restrict__ base, int64_t stride, int m
     return m;
 }
 
+// Columns 0 and 1 of an FP32 link: components 0..15, the four float4 planes
+// (the float2 plane holds only column 2 of row 2).
+template <typename C>
+LGF_DEV M3<C> load_link_cols01(const float* __restrict__ base, int64_t stride, int mu, int s) {
+    M3<C> m;
+    const float* p = base + (int64_t)mu * 18 * stride;
+    const uint32_t o4 = (uint32_t)s * 16u;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const vf4 v = ld<vf4>(p + (int64_t)k * 4 * stride, o4);
+        m.v[4 * k] = (C)v.x;
+        m.v[4 * k + 1] = (C)v.y;
+        m.v[4 * k + 2] = (C)v.z;
+        m.v[4 * k + 3] = (C)v.w;
+    }
+    return m;
+}
+
 template <typename S, typename C>
 LGF_DEV void store_link_rows01(S* __restrict__ base, int64_t stride, int mu, int s, const M3<C>& m) {
     S* p = base + (int64_t)mu * 18 * stride;

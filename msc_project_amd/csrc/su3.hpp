@@ -359,6 +359,78 @@ LGF_DEV M3<R> mul_dn_su3(const M3<R>& a, const M3<R>& b) {
     return o;
 }
 
+// Partial products whose result is only ever the left factor of one of the
+// SU(3) products above (which read rows 0, 1 of a left factor of mul_nn_su3
+// and mul_nd_su3, and columns 0, 1 of a left factor of mul_dn_su3): 72 FMAs.
+// The entries they do not compute are left unset.
+//   rows 0, 1 of a b        (a: rows 0, 1; b: whole)
+//   rows 0, 1 of a b^dag    (a: rows 0, 1; b: whole)
+//   rows 0, 1 of a^dag b^dag (a: columns 0, 1; b: whole)
+//   columns 0, 1 of a b     (a: whole; b: columns 0, 1)
+#define LGF_PART_BODY(RMAX, CMAX, AR, AI, BR, BI)                   \
+    M3<R> o;                                                         \
+    _Pragma("unroll") for (int r = 0; r < RMAX; ++r)                 \
+    _Pragma("unroll") for (int c = 0; c < CMAX; ++c) {               \
+        R x = R(0), y = R(0);                                        \
+        _Pragma("unroll") for (int k = 0; k < 3; ++k) {              \
+            LGF_CMAC(x, y, AR, AI, BR, BI)                           \
+        }                                                            \
+        o.v[2 * (3 * r + c)] = x;                                    \
+        o.v[2 * (3 * r + c) + 1] = y;                                \
+    }                                                                \
+    return o;
+
+template <typename R>
+LGF_DEV M3<R> mul_nn_r01(const M3<R>& a, const M3<R>& b) {
+    LGF_PART_BODY(2, 3, a.v[2 * (3 * r + k)], a.v[2 * (3 * r + k) + 1], b.v[2 * (3 * k + c)], b.v[2 * (3 * k + c) + 1])
+}
+template <typename R>
+LGF_DEV M3<R> mul_nd_r01(const M3<R>& a, const M3<R>& b) {
+    LGF_PART_BODY(2, 3, a.v[2 * (3 * r + k)], a.v[2 * (3 * r + k) + 1], b.v[2 * (3 * c + k)], -b.v[2 * (3 * c + k) + 1])
+}
+template <typename R>
+LGF_DEV M3<R> mul_dd_r01(const M3<R>& a, const M3<R>& b) {
+    // conj(a_kr) conj(b_ck) = (ar br - ai bi) - i (ar bi + ai br)
+    LGF_PART_BODY(2, 3, a.v[2 * (3 * k + r)], -a.v[2 * (3 * k + r) + 1], b.v[2 * (3 * c + k)], -b.v[2 * (3 * c + k) + 1])
+}
+template <typename R>
+LGF_DEV M3<R> mul_nn_c01(const M3<R>& a, const M3<R>& b) {
+    LGF_PART_BODY(3, 2, a.v[2 * (3 * r + k)], a.v[2 * (3 * r + k) + 1], b.v[2 * (3 * k + c)], b.v[2 * (3 * k + c) + 1])
+}
+#undef LGF_PART_BODY
+
+// y = w p, y += w p, y = w p^dag, y += w p^dag
+template <typename R>
+LGF_DEV void set_scaled(M3<R>& y, R w, const M3<R>& p) {
+#pragma unroll
+    for (int i = 0; i < 18; ++i) y.v[i] = w * p.v[i];
+}
+template <typename R>
+LGF_DEV void axpy(M3<R>& y, R w, const M3<R>& p) {
+#pragma unroll
+    for (int i = 0; i < 18; ++i) y.v[i] += w * p.v[i];
+}
+template <typename R>
+LGF_DEV void set_scaled_dag(M3<R>& y, R w, const M3<R>& p) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            y.v[2 * (3 * r + c)] = w * p.v[2 * (3 * c + r)];
+            y.v[2 * (3 * r + c) + 1] = -w * p.v[2 * (3 * c + r) + 1];
+        }
+}
+template <typename R>
+LGF_DEV void axpy_dag(M3<R>& y, R w, const M3<R>& p) {
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            y.v[2 * (3 * r + c)] += w * p.v[2 * (3 * c + r)];
+            y.v[2 * (3 * r + c) + 1] -= w * p.v[2 * (3 * c + r) + 1];
+        }
+}
+
 // Products of SU(3) matrices in the arithmetic of the mixed-precision path
 // (float: links projected onto SU(3) every flow step) use the cross-product
 // row; in double, the parity path, the full product as the reference.
