@@ -93,16 +93,19 @@ const char* lgf_ctx_last_error(const lgf_ctx* ctx);
 int lgf_ctx_stream(lgf_ctx* ctx, void** hip_stream);
 int lgf_ctx_synchronize(lgf_ctx* ctx);
 /* RK3 stage-kernel variant (register budget x workgroup shape) used by this
- * context for one precision; 0 = default.  Mixed-precision variants below
- * LGF_FP32_SHARED_PLAQUETTE_VARIANTS compute each plaquette once for the
- * four links of a site; the others sum each link's staples in the FP64
- * path's order.  Of the shared-plaquette variants, those from
- * LGF_FP32_COMPACT_VARIANTS on also share the backward links and rebuild
- * row 2 of the shared links from rows 0 and 1.  Variants of one of these
- * three groups compute bitwise identical links; the groups agree to FP32
- * rounding. */
+ * context for one precision; 0 = default.  Variants below
+ * LGF_FP32_SHARED_PLAQUETTE_VARIANTS (mixed precision) and
+ * LGF_FP64_SHARED_PLAQUETTE_VARIANTS (FP64) compute each plaquette once for
+ * the four links of a site; the others sum each link's staples in the
+ * reference's order.  Of the mixed-precision shared-plaquette variants,
+ * those from LGF_FP32_COMPACT_VARIANTS on also share the backward links and
+ * rebuild row 2 of the shared links from rows 0 and 1.  Variants of one
+ * group compute bitwise identical links; the groups agree to rounding in the
+ * path's precision.  For FP64 the autotuners keep the group of the
+ * context's current variant, so tuning never changes an FP64 result. */
 #define LGF_FP32_SHARED_PLAQUETTE_VARIANTS 8
 #define LGF_FP32_COMPACT_VARIANTS 6
+#define LGF_FP64_SHARED_PLAQUETTE_VARIANTS 2
 int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
 int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
 int lgf_num_stage_variants(int precision, int* n);
@@ -132,7 +135,8 @@ int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes);
  * nsteps RK3 steps on a seeded lattice of `shape` (each pair three times, its
  * fastest run counting), and make the fastest pair this context's tile order
  * and its stage variant for `precision`; ms_per_step (optional) gets one
- * entry per variant, at the chosen order.  No counterpart in the reference
+ * entry per variant, at the chosen order (HUGE_VAL for FP64 variants outside
+ * the group of the context's variant, which are not timed).  No counterpart in the reference
  * (its flow has no launch parameters to choose). */
 int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best, double* ms_per_step);
 /* the same search on `nshapes` seeded lattices (shapes[4*i .. 4*i+3])

@@ -960,6 +960,13 @@ static const size_t kGroupSearch = size_t(224) << 20;
 static const size_t kGroupCaps[] = {0, size_t(128) << 20, size_t(192) << 20, size_t(224) << 20,
                                     size_t(256) << 20};
 
+// FP64 variants of another group than the context's current one sum the
+// staples in another order: the autotuners leave them out, so that tuning
+// never changes an FP64 result
+static bool tunable(int precision, int current, int v) {
+    return precision != LGF_FP64 || stage_variant_group(precision, v) == stage_variant_group(precision, current);
+}
+
 static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int precision, int nsteps, int* best,
                            double* ms_per_step) {
     static const int orders[] = {0, 2, 4, 8, 404, 408};
@@ -985,12 +992,13 @@ static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int pre
     // lattices) in cache groups
     const int ncaps = fields.size() > 1 ? 2 : 1;
     const size_t search_caps[2] = {0, kGroupSearch};
-    std::vector<double> ms(size_t(ncaps) * norders * nv, 1e300);
+    std::vector<double> ms(size_t(ncaps) * norders * nv, HUGE_VAL);
     for (int ci = 0; ci < ncaps && !rc; ++ci) {
         ctx->group_bytes = search_caps[ci];
         for (int oi = 0; oi < norders && !rc; ++oi) {
             ctx->tile_block = orders[oi];
             for (int v = 0; v < nv && !rc; ++v) {
+                if (!tunable(precision, keep_variant, v)) continue;
                 ctx->stage_variant[precision] = v;
                 rc = time_batch(ctx, fields, eps, nsteps, e0, e1, &ms[(size_t(ci) * norders + oi) * nv + v]);
             }

@@ -147,14 +147,22 @@ struct PlaqLds {
     C v[6][9][kTileSites];  // [plane][ta component][lane]: lane-contiguous rows, no bank conflicts
 };
 
-// The four links U_d(x) of a tile's sites in LDS (float4 planes of
-// components 0..15 and a float2 plane of 16, 17, as in global memory).  With
-// LOC each direction wave loads its own link once and leaves it here; the
-// plaquettes read the rows they need (0 and 1 of the left factor of each
-// half-plaquette) and the wave rereads its whole link for the update.
-struct LocLds {
+// The four links U_d(x) of a tile's sites in LDS, in the plane layout of
+// global memory: FP32 float4 planes of components 0..15 and a float2 plane
+// of 16, 17; FP64 nine double2 planes.  With LOC each direction wave loads
+// its own link once and leaves it here; the plaquettes read the rows they
+// need (0 and 1 of the left factor of each half-plaquette in FP32, all of it
+// in FP64) and the wave rereads its whole link for the update.
+template <typename C>
+struct LocLds;
+template <>
+struct LocLds<float> {
     vf4 q[4][4][kTileSites];
     vf2 r[4][kTileSites];
+};
+template <>
+struct LocLds<double> {
+    vd2 q[4][9][kTileSites];
 };
 
 // Everything of link (x,mu) after its flow gradient is known (nine-real
@@ -182,9 +190,11 @@ LGF_DEV void stage_finish(const C* z, const M3<C>& Umu, S* __restrict__ Un, S* _
 #pragma unroll
         for (int i = 0; i < 18; ++i) E.v[i] = eps * ((C)(3.0 / 4.0) * Z.v[i] - X1.v[i]);
     }
-    const M3<C> ex = exp_ah_series<C>(E);
+    // the exponential by its power series: 13 terms with FP64 links (the
+    // parity path), 7 in the mixed-precision one
+    const M3<C> ex = exp_ah_series<C, sizeof(S) == 8 ? 13 : 7>(E);
     const M3<C> Unew = su3_nn(ex, Umu);
-    if constexpr (STAGE == 2) {
+    if constexpr (STAGE == 2 && sizeof(S) == 4) {
         // once per RK3 step: project back onto SU(3) in double precision
         M3<double> Ud;
 #pragma unroll
@@ -196,23 +206,38 @@ LGF_DEV void stage_finish(const C* z, const M3<C>& Umu, S* __restrict__ Un, S* _
     }
 }
 
-template <bool ON>
+template <bool ON, typename C>
 struct LocSlot {
-    LocLds b;
-    LGF_DEV LocLds* ptr() { return &b; }
+    LocLds<C> b;
+    LGF_DEV LocLds<C>* ptr() { return &b; }
 };
-template <>
-struct LocSlot<false> {
-    LGF_DEV LocLds* ptr() { return nullptr; }
+template <typename C>
+struct LocSlot<false, C> {
+    LGF_DEV LocLds<C>* ptr() { return nullptr; }
 };
 
-LGF_DEV void loc_store(LocLds& l, int d, int lane, const M3<float>& m) {
+LGF_DEV void loc_store(LocLds<float>& l, int d, int lane, const M3<float>& m) {
 #pragma unroll
     for (int k = 0; k < 4; ++k) l.q[d][k][lane] = vf4{m.v[4 * k], m.v[4 * k + 1], m.v[4 * k + 2], m.v[4 * k + 3]};
     l.r[d][lane] = vf2{m.v[16], m.v[17]};
 }
 
-LGF_DEV M3<float> loc_load(const LocLds& l, int d, int lane) {
+LGF_DEV void loc_store(LocLds<double>& l, int d, int lane, const M3<double>& m) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) l.q[d][k][lane] = vd2{m.v[2 * k], m.v[2 * k + 1]};
+}
+
+LGF_DEV M3<double> loc_load(const LocLds<double>& l, int d, int lane) {
+    M3<double> m;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const vd2 v = l.q[d][k][lane];
+        m.v[2 * k] = v.x, m.v[2 * k + 1] = v.y;
+    }
+    return m;
+}
+
+LGF_DEV M3<float> loc_load(const LocLds<float>& l, int d, int lane) {
     M3<float> m;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
@@ -226,7 +251,7 @@ LGF_DEV M3<float> loc_load(const LocLds& l, int d, int lane) {
 
 // U_d(x) from global memory or, with LOC, from LDS
 template <bool LOC, typename S, typename C>
-LGF_DEV M3<C> local_link(const S* __restrict__ U, int64_t st, int d, int s, const LocLds* loc, int lane) {
+LGF_DEV M3<C> local_link(const S* __restrict__ U, int64_t st, int d, int s, const LocLds<C>* loc, int lane) {
     if constexpr (LOC) return loc_load(*loc, d, lane);
     else return load_link<S, C>(U, st, d, s);
 }
@@ -234,7 +259,7 @@ LGF_DEV M3<C> local_link(const S* __restrict__ U, int64_t st, int d, int s, cons
 // -1/2 TA(P_ab(x)) of one plane into LDS; returns the last value computed
 template <typename S, typename C, int A, int B, bool LOC>
 LGF_DEV C plaquette_to_lds(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, const Geo& g, int lane,
-                           const LocLds* loc) {
+                           const LocLds<C>* loc) {
     const int s = g.s;
     const int fa = nb_off(g, A, +1), fb = nb_off(g, B, +1);
     const M3<C> h1 = su3_nn(local_link<LOC, S, C>(U, st, A, s, loc, lane), load_link<S, C>(U, st, B, s + fa));
@@ -262,7 +287,7 @@ LGF_DEV void after(Geo& g, C dep) {
 // this wave's share of the site's six plaquettes; returns the last value
 // computed
 template <typename S, typename C, int MU, bool ORDERED, bool LOC>
-LGF_DEV C plaquettes_of(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, Geo g, int lane, const LocLds* loc) {
+LGF_DEV C plaquettes_of(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, Geo g, int lane, const LocLds<C>* loc) {
     C dep;
     if constexpr (MU == 0) {
         dep = plaquette_to_lds<S, C, 0, 1, LOC>(sp, U, st, g, lane, loc);
@@ -284,7 +309,7 @@ LGF_DEV C plaquettes_of(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, Geo
 // -1/2 TA(U_mu sum_nu lower_nu) in the nine-real form.
 template <typename S, typename C, int MU, bool ORDERED, bool LOC>
 LGF_DEV void stage_pre_pl(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, Geo g, int lane, C* z, M3<C>& Umu,
-                          const LocLds* loc) {
+                          const LocLds<C>* loc) {
     const C dep = plaquettes_of<S, C, MU, ORDERED, LOC>(sp, U, st, g, lane, loc);
     after<ORDERED>(g, dep);
     constexpr int N0 = MU == 0 ? 1 : 0, N1 = MU <= 1 ? 2 : 1, N2 = MU <= 2 ? 3 : 2;
@@ -317,10 +342,10 @@ LGF_DEV void add_planes(const PlaqLds<C>& sp, int mu, int lane, C* z) {
 template <typename S, typename C, int STAGE, int WPE, bool ORDERED, bool LOC, bool NTS = false>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_rk3_stage_pl(
     const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int ping) {
-    static_assert(std::is_same<S, float>::value && std::is_same<C, float>::value, "mixed-precision path only");
+    static_assert(std::is_same<S, C>::value, "arithmetic in the precision of the links");
     __shared__ PlaqLds<C> sp;
-    __shared__ LocSlot<LOC> lslot;
-    LocLds* loc = lslot.ptr();
+    __shared__ LocSlot<LOC, C> lslot;
+    LocLds<C>* loc = lslot.ptr();
     const int t = xcd_remap(blockIdx.x, gridDim.x);
     const int2 tm = tiles[t];
     const FieldDesc& d = fds[tm.x];
@@ -1270,19 +1295,16 @@ static hipError_t stage_launch_pb(int stage, const FieldDesc* fds, const int2* t
     return hipGetLastError();
 }
 
-template <int WPE, bool ORDERED, bool LOC = false, bool NTS = false>
+template <int WPE, bool ORDERED, bool LOC = false, bool NTS = false, typename S = float>
 static hipError_t stage_launch_pl(int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,
                                   hipStream_t stream) {
     dim3 grid(ntiles), block(kTileSites, 4);
     if (stage == 0)
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 0, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles,
-                           ping);
+        hipLaunchKernelGGL((k_rk3_stage_pl<S, S, 0, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles, ping);
     else if (stage == 1)
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 1, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles,
-                           ping);
+        hipLaunchKernelGGL((k_rk3_stage_pl<S, S, 1, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles, ping);
     else
-        hipLaunchKernelGGL((k_rk3_stage_pl<float, float, 2, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles,
-                           ping);
+        hipLaunchKernelGGL((k_rk3_stage_pl<S, S, 2, WPE, ORDERED, LOC, NTS>), grid, block, 0, stream, fds, tiles, ping);
     return hipGetLastError();
 }
 
@@ -1292,17 +1314,27 @@ static hipError_t stage_launch_pl(int stage, const FieldDesc* fds, const int2* t
 // 6 and 7 the shared-plaquette kernel that also shares the backward links
 // (7 with non-temporal stores), 8..13 the per-link kernel.  Variant 0 is the default (a budget the kernel
 // fits without spilling); lgf_autotune picks the fastest.
-int num_stage_variants(int prec) { return prec == kPrecFP64 ? 5 : 14; }
+// FP64 variants 0 and 1 are the shared-plaquette kernel (0 with the site's
+// links in LDS), 2..6 the per-link kernel, which sums each link's staples in
+// the reference's order.
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 7 : 14; }
+
+int stage_variant_group(int prec, int variant) {
+    if (prec == kPrecFP64) return variant < LGF_FP64_SHARED_PLAQUETTE_VARIANTS ? 0 : 1;
+    return variant < LGF_FP32_COMPACT_VARIANTS ? 0 : variant < LGF_FP32_SHARED_PLAQUETTE_VARIANTS ? 1 : 2;
+}
 
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream) {
     if (prec == kPrecFP64) {
         switch (variant) {
-            case 1: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
-            case 2: return stage_launch<double, double, 0, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
-            case 3: return stage_launch<double, double, 0, 3, 1>(stage, fds, tiles, ntiles, ping, stream);
-            case 4: return stage_launch<double, double, 0, 2, 1>(stage, fds, tiles, ntiles, ping, stream);
-            default: return stage_launch<double, double, 0, 2, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 1: return stage_launch_pl<2, true, false, false, double>(stage, fds, tiles, ntiles, ping, stream);
+            case 2: return stage_launch<double, double, 0, 2, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 3: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 4: return stage_launch<double, double, 0, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 5: return stage_launch<double, double, 0, 3, 1>(stage, fds, tiles, ntiles, ping, stream);
+            case 6: return stage_launch<double, double, 0, 2, 1>(stage, fds, tiles, ntiles, ping, stream);
+            default: return stage_launch_pl<2, true, true, false, double>(stage, fds, tiles, ntiles, ping, stream);
         }
     }
     switch (variant) {

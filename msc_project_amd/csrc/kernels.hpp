@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/lgf_flow.h"
 #include "lattice.hpp"
 
 namespace lgf {
@@ -13,6 +14,9 @@ constexpr int kCloverParts = 5;  // clover partials per tile: e, q, |q|, improve
 constexpr int kOutStride = 8;    // doubles per field in reduction outputs
 
 int num_stage_variants(int prec);
+// stage variants of one group compute bitwise identical links; groups agree
+// to rounding in the path's precision
+int stage_variant_group(int prec, int variant);
 hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const FieldDesc* fds, const int2* tiles,
                                     int ntiles, int ping, hipStream_t stream);
 hipError_t launch_rk3_stage(int prec, int stage, const FieldDesc* fds, const int2* tiles, int ntiles, int ping,

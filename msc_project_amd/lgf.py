@@ -27,9 +27,14 @@ MEASURE_FP64 = 1
 # row 2 of shared links (LGF_FP32_COMPACT_VARIANTS)
 FP32_SHARED_PLAQUETTE_VARIANTS = 8
 FP32_COMPACT_VARIANTS = 6
+# FP64 stage variants below this one share each site's plaquettes; the
+# others sum each link's staples in the reference's order
+# (LGF_FP64_SHARED_PLAQUETTE_VARIANTS)
+FP64_SHARED_PLAQUETTE_VARIANTS = 2
 # variant ranges whose links agree bit for bit
 FP32_VARIANT_GROUPS = ((0, FP32_COMPACT_VARIANTS), (FP32_COMPACT_VARIANTS, FP32_SHARED_PLAQUETTE_VARIANTS),
                        (FP32_SHARED_PLAQUETTE_VARIANTS, None))
+FP64_VARIANT_GROUPS = ((0, FP64_SHARED_PLAQUETTE_VARIANTS), (FP64_SHARED_PLAQUETTE_VARIANTS, None))
 SERVE_ALL = 0             # every admitted job steps in every round
 SERVE_SHORTEST_FIRST = 1  # least remaining work first, at least min_sites per round
 # lgf_observable order (include/lgf_flow.h)

@@ -99,9 +99,10 @@ def test_host_flow_config1(tmp_path, mode):
 @pytest.mark.parametrize("mode", ["flow", "batch"])
 def test_host_autotune_leaves_fp64_outputs_unchanged(tmp_path, mode):
     """--autotune only chooses launch parameters (tile order and stage
-    variant, every FP64 choice computes the same links and sums): the CSVs,
-    their full-precision twins and the flowed configurations are byte for
-    byte those of the untuned run"""
+    variant; FP64 tuning stays in the group of the default variant, whose
+    choices compute the same links and sums): the CSVs, their full-precision
+    twins and the flowed configurations are byte for byte those of the
+    untuned run"""
     d, shape, cfgs = config1_inputs()
     outs = []
     for tag, opts in (("plain", ()), ("tuned", ("--autotune=2",))):

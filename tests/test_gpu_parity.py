@@ -149,7 +149,8 @@ def test_flow_keeps_constant_abelian_field(ctx):
     shape = (6, 4, 4, 6)
     U, p, q = twisted_flux_field(shape, 1, -2)
     for prec, tol in ((lgf.FP64, 1e-13), (lgf.FP32, 2e-6)):
-        variants = [0, lgf.FP32_COMPACT_VARIANTS, lgf.FP32_SHARED_PLAQUETTE_VARIANTS] if prec == lgf.FP32 else [0]
+        variants = ([0, lgf.FP32_COMPACT_VARIANTS, lgf.FP32_SHARED_PLAQUETTE_VARIANTS] if prec == lgf.FP32
+                    else [0, lgf.FP64_SHARED_PLAQUETTE_VARIANTS])
         for v in variants:
             ctx.set_stage_variant(prec, v)
             f = lgf.Field(ctx, shape, prec)
@@ -364,10 +365,11 @@ def test_fp32_tracks_fp64(ctx, config1, variant):
 
 @pytest.mark.parametrize("shape", [(8, 4, 6, 4), (6, 4, 2, 2)])
 def test_stage_variants_agree_and_autotune(ctx, shape):
-    """every stage-kernel variant computes the same links (the mixed-precision
-    shared-plaquette and per-link kernels, which sum the staples in different
-    orders, to FP32 rounding); the autotuner picks one of them.  6x4x2x2 has
-    a ragged last tile (96 sites)."""
+    """every stage-kernel variant computes the same links (the shared-plaquette
+    and per-link kernels, which sum the staples in different orders, to
+    rounding in the path's precision); the autotuner picks one of them, for
+    FP64 within the group of the current variant.  6x4x2x2 has a ragged last
+    tile (96 sites)."""
     U = O.generate(shape, 6.0, seed=31, start="hot", sweeps=1)
     for prec in (lgf.FP64, lgf.FP32):
         outs = []
@@ -380,15 +382,25 @@ def test_stage_variants_agree_and_autotune(ctx, shape):
             f.close()
         # variants in one group compute the same links bit for bit, the
         # groups agree to FP32 rounding
-        groups = lgf.FP32_VARIANT_GROUPS if prec == lgf.FP32 else ((0, None),)
+        groups = lgf.FP32_VARIANT_GROUPS if prec == lgf.FP32 else lgf.FP64_VARIANT_GROUPS
+        tol = 2e-6 if prec == lgf.FP32 else 1e-14
         for a, b in groups:
             grp = outs[a:b]
             for o in grp[1:]:
                 assert np.array_equal(o, grp[0]), (prec, a, b)
-            assert np.max(np.abs(grp[0] - outs[0])) < 2e-6, (prec, a)
-        best, ms = ctx.autotune(shape, prec, 2)
-        assert 0 <= best < len(ms) and all(m > 0 for m in ms)
-        assert ctx.stage_variant(prec) == best
+            assert np.max(np.abs(grp[0] - outs[0])) < tol, (prec, a)
+        for start in ((0, lgf.FP64_SHARED_PLAQUETTE_VARIANTS) if prec == lgf.FP64 else (0,)):
+            ctx.set_stage_variant(prec, start)
+            best, ms = ctx.autotune(shape, prec, 2)
+            assert 0 <= best < len(ms) and all(m > 0 for m in ms)
+            assert ctx.stage_variant(prec) == best
+            if prec == lgf.FP64:
+                # FP64 tuning stays in the group it started from; the others are not timed
+                grp = [g for g in lgf.FP64_VARIANT_GROUPS if g[0] == start][0]
+                inside = range(len(ms))[grp[0]:grp[1]]
+                assert best in inside
+                assert all(np.isinf(m) for v, m in enumerate(ms) if v not in inside)
+                assert all(np.isfinite(ms[v]) for v in inside)
         assert ctx.tile_order() in (0, 2, 4, 8, 404, 408)
         ctx.set_stage_variant(prec, 0)
     # the traversal order changes neither links nor observables
@@ -408,13 +420,17 @@ def test_stage_variants_agree_and_autotune(ctx, shape):
 def test_autotune_batch(ctx):
     """The batch autotuner times every (tile order, variant) pair on a mix of
     shapes flowed as one batch and keeps the fastest; the returned times are
-    those of the chosen order, so the chosen variant is their minimum.  A bad
-    shape or an empty list is refused and leaves the context as it was."""
+    those of the chosen order, so the chosen variant is their minimum.  FP64
+    variants outside the group of the context's variant are not timed.  A
+    bad shape or an empty list is refused and leaves the context as it
+    was."""
     shapes = [(4, 4, 4, 8), (6, 4, 2, 2), (8, 8, 4, 4)]
     for prec in (lgf.FP64, lgf.FP32):
         best, ms = ctx.autotune_batch(shapes, prec, 2)
         assert len(ms) == lgf.num_stage_variants(prec)
-        assert 0 <= best < len(ms) and all(0 < m < 1e3 for m in ms)
+        timed = range(lgf.FP64_SHARED_PLAQUETTE_VARIANTS) if prec == lgf.FP64 else range(len(ms))
+        assert 0 <= best < len(ms) and all(0 < ms[v] < 1e3 for v in timed)
+        assert all(np.isinf(m) for v, m in enumerate(ms) if v not in timed)
         assert ms[best] == min(ms)
         assert ctx.stage_variant(prec) == best
         assert ctx.tile_order() in (0, 2, 4, 8, 404, 408)
