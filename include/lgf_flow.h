@@ -105,7 +105,7 @@ int lgf_ctx_synchronize(lgf_ctx* ctx);
  * context's current variant, so tuning never changes an FP64 result. */
 #define LGF_FP32_SHARED_PLAQUETTE_VARIANTS 8
 #define LGF_FP32_COMPACT_VARIANTS 6
-#define LGF_FP64_SHARED_PLAQUETTE_VARIANTS 2
+#define LGF_FP64_SHARED_PLAQUETTE_VARIANTS 3
 int lgf_ctx_set_stage_variant(lgf_ctx* ctx, int precision, int variant);
 int lgf_ctx_get_stage_variant(lgf_ctx* ctx, int precision, int* variant);
 int lgf_num_stage_variants(int precision, int* n);

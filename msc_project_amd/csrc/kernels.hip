@@ -1314,10 +1314,10 @@ static hipError_t stage_launch_pl(int stage, const FieldDesc* fds, const int2* t
 // 6 and 7 the shared-plaquette kernel that also shares the backward links
 // (7 with non-temporal stores), 8..13 the per-link kernel.  Variant 0 is the default (a budget the kernel
 // fits without spilling); lgf_autotune picks the fastest.
-// FP64 variants 0 and 1 are the shared-plaquette kernel (0 with the site's
-// links in LDS), 2..6 the per-link kernel, which sums each link's staples in
-// the reference's order.
-int num_stage_variants(int prec) { return prec == kPrecFP64 ? 7 : 14; }
+// FP64 variants 0..2 are the shared-plaquette kernel (0 and 2 with the
+// site's links in LDS, 2 with non-temporal stores), 3..7 the per-link
+// kernel, which sums each link's staples in the reference's order.
+int num_stage_variants(int prec) { return prec == kPrecFP64 ? 8 : 14; }
 
 int stage_variant_group(int prec, int variant) {
     if (prec == kPrecFP64) return variant < LGF_FP64_SHARED_PLAQUETTE_VARIANTS ? 0 : 1;
@@ -1329,11 +1329,12 @@ hipError_t launch_rk3_stage_variant(int prec, int variant, int stage, const Fiel
     if (prec == kPrecFP64) {
         switch (variant) {
             case 1: return stage_launch_pl<2, true, false, false, double>(stage, fds, tiles, ntiles, ping, stream);
-            case 2: return stage_launch<double, double, 0, 2, 4>(stage, fds, tiles, ntiles, ping, stream);
-            case 3: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
-            case 4: return stage_launch<double, double, 0, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
-            case 5: return stage_launch<double, double, 0, 3, 1>(stage, fds, tiles, ntiles, ping, stream);
-            case 6: return stage_launch<double, double, 0, 2, 1>(stage, fds, tiles, ntiles, ping, stream);
+            case 2: return stage_launch_pl<2, true, true, true, double>(stage, fds, tiles, ntiles, ping, stream);
+            case 3: return stage_launch<double, double, 0, 2, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 4: return stage_launch<double, double, 0, 3, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 5: return stage_launch<double, double, 0, 4, 4>(stage, fds, tiles, ntiles, ping, stream);
+            case 6: return stage_launch<double, double, 0, 3, 1>(stage, fds, tiles, ntiles, ping, stream);
+            case 7: return stage_launch<double, double, 0, 2, 1>(stage, fds, tiles, ntiles, ping, stream);
             default: return stage_launch_pl<2, true, true, false, double>(stage, fds, tiles, ntiles, ping, stream);
         }
     }

@@ -243,6 +243,11 @@ LGF_DEV void store_link(S* __restrict__ base, int64_t stride, int mu, int s, con
             st_(p + (int64_t)k * 4 * stride, o4,
                 vf4{(float)m.v[4 * k], (float)m.v[4 * k + 1], (float)m.v[4 * k + 2], (float)m.v[4 * k + 3]});
         st_(p + 16 * stride, o2, vf2{(float)m.v[16], (float)m.v[17]});
+    } else if constexpr (NT) {
+        const uint32_t o = (uint32_t)s * 16u;
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+            st_nt(p + (int64_t)k * 2 * stride, o, vd2{(double)m.v[2 * k], (double)m.v[2 * k + 1]});
     } else {
         const uint32_t o = (uint32_t)s * 16u;
 #pragma unroll
@@ -306,6 +311,12 @@ LGF_DEV void store_ta(S* __restrict__ base, int64_t stride, int mu, int s, const
         st_(p, (uint32_t)s * 16u, vf4{(float)m.v[1], (float)m.v[9], (float)m.v[17], (float)m.v[2]});
         st_(p + 4 * stride, (uint32_t)s * 16u, vf4{(float)m.v[3], (float)m.v[4], (float)m.v[5], (float)m.v[10]});
         st_(p + 8 * stride, (uint32_t)s * 4u, (float)m.v[11]);
+    } else if constexpr (NT) {
+        st_nt(p, (uint32_t)s * 16u, vd2{(double)m.v[1], (double)m.v[9]});
+        st_nt(p + 2 * stride, (uint32_t)s * 16u, vd2{(double)m.v[17], (double)m.v[2]});
+        st_nt(p + 4 * stride, (uint32_t)s * 16u, vd2{(double)m.v[3], (double)m.v[4]});
+        st_nt(p + 6 * stride, (uint32_t)s * 16u, vd2{(double)m.v[5], (double)m.v[10]});
+        st_nt(p + 8 * stride, (uint32_t)s * 8u, (double)m.v[11]);
     } else {
         st_(p, (uint32_t)s * 16u, vd2{(double)m.v[1], (double)m.v[9]});
         st_(p + 2 * stride, (uint32_t)s * 16u, vd2{(double)m.v[17], (double)m.v[2]});

@@ -30,7 +30,7 @@ FP32_COMPACT_VARIANTS = 6
 # FP64 stage variants below this one share each site's plaquettes; the
 # others sum each link's staples in the reference's order
 # (LGF_FP64_SHARED_PLAQUETTE_VARIANTS)
-FP64_SHARED_PLAQUETTE_VARIANTS = 2
+FP64_SHARED_PLAQUETTE_VARIANTS = 3
 # variant ranges whose links agree bit for bit
 FP32_VARIANT_GROUPS = ((0, FP32_COMPACT_VARIANTS), (FP32_COMPACT_VARIANTS, FP32_SHARED_PLAQUETTE_VARIANTS),
                        (FP32_SHARED_PLAQUETTE_VARIANTS, None))
