@@ -17,6 +17,10 @@ namespace lgf {
 // with exactly mirrored lower triangle): Q^2 is Hermitian with a real
 // diagonal, so only its diagonal and upper triangle are computed (54 FMAs
 // instead of 108) and the lower triangle is their conjugate.
+// Q's imaginary diagonal is zero and never read: the exponents here are
+// anti-Hermitian with an exactly zero real diagonal, and x + a * 0 is not
+// folded away without fast-math.  The terms that remain are summed in the
+// order of the full product, so the result is the same to the last bit.
 template <typename R>
 LGF_DEV void herm_square(const M3<R>& Q, M3<R>& Q2, R& tr3) {
 #pragma unroll
@@ -25,7 +29,7 @@ LGF_DEV void herm_square(const M3<R>& Q, M3<R>& Q2, R& tr3) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             d += Q.v[2 * (3 * r + k)] * Q.v[2 * (3 * r + k)];
-            d += Q.v[2 * (3 * r + k) + 1] * Q.v[2 * (3 * r + k) + 1];
+            if (k != r) d += Q.v[2 * (3 * r + k) + 1] * Q.v[2 * (3 * r + k) + 1];
         }
         Q2.v[8 * r] = d;
         Q2.v[8 * r + 1] = R(0);
@@ -37,8 +41,17 @@ LGF_DEV void herm_square(const M3<R>& Q, M3<R>& Q2, R& tr3) {
             R x = R(0), y = R(0);
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                LGF_CMAC(x, y, Q.v[2 * (3 * r + k)], Q.v[2 * (3 * r + k) + 1], Q.v[2 * (3 * k + c)],
-                         Q.v[2 * (3 * k + c) + 1])
+                const R ar = Q.v[2 * (3 * r + k)], ai = Q.v[2 * (3 * r + k) + 1];
+                const R br = Q.v[2 * (3 * k + c)], bi = Q.v[2 * (3 * k + c) + 1];
+                if (k == r) {  // a real
+                    x += ar * br;
+                    y += ar * bi;
+                } else if (k == c) {  // b real
+                    x += ar * br;
+                    y += ai * br;
+                } else {
+                    LGF_CMAC(x, y, ar, ai, br, bi)
+                }
             }
             Q2.v[2 * (3 * r + c)] = x;
             Q2.v[2 * (3 * r + c) + 1] = y;
@@ -71,11 +84,13 @@ LGF_DEV void herm_square(const M3<R>& Q, M3<R>& Q2, R& tr3) {
 template <typename R, int NTERMS = 7>
 LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
     constexpr float C1MAX = 0.02f;
+    // Q = -i X is Hermitian; X's real diagonal is zero, so Q's imaginary
+    // diagonal is left zero and never read
     M3<R> Q;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
         Q.v[2 * i] = X.v[2 * i + 1];
-        Q.v[2 * i + 1] = -X.v[2 * i];
+        Q.v[2 * i + 1] = i % 4 == 0 ? R(0) : -X.v[2 * i];
     }
     M3<R> Q2;
     R tr3;
@@ -117,16 +132,18 @@ LGF_DEV M3<R> exp_ah_series(const M3<R>& X) {
     M3<R> o;
 #pragma unroll
     for (int i = 0; i < 9; ++i) {
-        R qr = Q.v[2 * i], qi = Q.v[2 * i + 1], pr = Q2.v[2 * i], pi = Q2.v[2 * i + 1];
-        o.v[2 * i] = qr * f1r - qi * f1i + pr * f2r - pi * f2i;
-        o.v[2 * i + 1] = qr * f1i + qi * f1r + pr * f2i + pi * f2r;
+        const R qr = Q.v[2 * i], pr = Q2.v[2 * i];
+        if (i % 4 == 0) {  // diagonal: Q and Q^2 real there
+            o.v[2 * i] = qr * f1r + pr * f2r;
+            o.v[2 * i + 1] = qr * f1i + pr * f2i;
+            o.v[2 * i] += f0r;
+            o.v[2 * i + 1] += f0i;
+        } else {
+            const R qi = Q.v[2 * i + 1], pi = Q2.v[2 * i + 1];
+            o.v[2 * i] = qr * f1r - qi * f1i + pr * f2r - pi * f2i;
+            o.v[2 * i + 1] = qr * f1i + qi * f1r + pr * f2i + pi * f2r;
+        }
     }
-    o.v[0] += f0r;
-    o.v[1] += f0i;
-    o.v[8] += f0r;
-    o.v[9] += f0i;
-    o.v[16] += f0r;
-    o.v[17] += f0i;
     for (int s = 0; s < k; ++s) o = mul_nn(o, o);
     return o;
 }

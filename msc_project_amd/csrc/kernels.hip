@@ -11,9 +11,11 @@
 //              eps X1 and eps (3/4 Z2 - X1), which is the same integrator
 //              with one accumulator (also the form of the thesis algorithm,
 //              doc/thesis.tex:2569-2600).
-// k_rk3_stage_pl  the same stage for the mixed-precision path with each
-//              site's six plaquettes computed once for its four links (the
-//              four direction waves of a workgroup share them in LDS).
+// k_rk3_stage_pl  the same stage with each site's six plaquettes computed
+//              once for its four links (the four direction waves of a
+//              workgroup share them in LDS), in both precisions;
+//              k_rk3_stage_pb the mixed-precision form that also shares the
+//              backward links.
 // k_clover     1x1 clover F (energy density) and the improved F (19/9 1x1 +
 //              1/36 2x2 - 32/45 1x2 + 1/15 1x3, LGF/Field/SU3_field.cpp:533-797)
 //              for the plane pairs of the topological charge; per-site q(x),
@@ -124,7 +126,7 @@ __global__ __launch_bounds__(64 * BY) __attribute__((amdgpu_waves_per_eu(WPE, 8)
 }
 
 // ---------------------------------------------------------------------------
-// RK3 stage with shared plaquettes (mixed-precision path)
+// RK3 stage with shared plaquettes
 // ---------------------------------------------------------------------------
 // The staple sum of link (x,mu) has an upper term in each plane (mu,nu),
 // U_mu(x) * upper = P_{mu nu}(x) = U_mu(x) U_nu(x+mu) U_mu(x+nu)^dag U_nu(x)^dag,
@@ -144,7 +146,7 @@ constexpr int plane_index(int a, int b) { return a == 0 ? b - 1 : a == 1 ? b + 1
 
 template <typename C>
 struct PlaqLds {
-    C v[6][9][kTileSites];  // [plane][ta component][lane]: lane-contiguous rows, no bank conflicts
+    C v[6][9][kTileSites];  // [plane][folded component][lane]: lane-contiguous rows, no bank conflicts
 };
 
 // The four links U_d(x) of a tile's sites in LDS, in the plane layout of
@@ -256,7 +258,21 @@ LGF_DEV M3<C> local_link(const S* __restrict__ U, int64_t st, int d, int s, cons
     else return load_link<S, C>(U, st, d, s);
 }
 
-// -1/2 TA(P_ab(x)) of one plane into LDS; returns the last value computed
+// The projections are summed in the folded form of su3.hpp (Ta9: the
+// imaginary diagonal and the upper triangle of W - W^dag, each product term
+// going into one running sum), and the trace removal and the -1/2 scale are
+// applied once to the link's sum (minus_half_ta).
+
+// the nine reals of -1/2 TA(W) from the traceless folded form of W
+template <typename C>
+LGF_DEV void minus_half_ta(C* z) {
+#pragma unroll
+    for (int d = 0; d < 3; ++d) z[d] = -z[d];
+#pragma unroll
+    for (int k = 3; k < 9; ++k) z[k] *= C(-0.5);
+}
+
+// the folded P_ab(x) of one plane into LDS; returns the last value computed
 template <typename S, typename C, int A, int B, bool LOC>
 LGF_DEV C plaquette_to_lds(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, const Geo& g, int lane,
                            const LocLds<C>* loc) {
@@ -264,12 +280,13 @@ LGF_DEV C plaquette_to_lds(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, 
     const int fa = nb_off(g, A, +1), fb = nb_off(g, B, +1);
     const M3<C> h1 = su3_nn(local_link<LOC, S, C>(U, st, A, s, loc, lane), load_link<S, C>(U, st, B, s + fa));
     const M3<C> h2 = su3_nn(local_link<LOC, S, C>(U, st, B, s, loc, lane), load_link<S, C>(U, st, A, s + fb));
-    C t[9];
-    ta_pack(ta_of_product_nd(h1, h2, C(-0.5)), t);
+    Ta9<C> w;
+    set_zero(w);
+    mul_nd_acc_ta(w, h1, h2);
     constexpr int p = plane_index(A, B);
 #pragma unroll
-    for (int k = 0; k < 9; ++k) sp.v[p][k][lane] = t[k];
-    return t[8];
+    for (int k = 0; k < 9; ++k) sp.v[p][k][lane] = w.v[k];
+    return w.v[8];
 }
 
 // Sequencing of the sections of a link update.  Loads of read-only link
@@ -322,7 +339,11 @@ LGF_DEV void stage_pre_pl(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, G
     after<ORDERED>(g, A.v[17]);
     lower_term<S, C, MU, N2>(A, U, st, g, fmu);
     Umu = local_link<LOC, S, C>(U, st, MU, g.s, loc, lane);
-    ta_pack(ta_of_product(Umu, A, C(-0.5)), z);
+    Ta9<C> w;
+    set_zero(w);
+    mul_nn_acc_ta(w, Umu, A);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) z[k] = w.v[k];
 }
 
 // z += the projections of link mu's three planes (sign - where mu is the
@@ -376,6 +397,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     }
     __syncthreads();
     add_planes(sp, mu, lane, z);
+    Ta9<C> zt;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) zt.v[k] = z[k];
+    make_traceless(zt);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) z[k] = zt.v[k];
+    minus_half_ta(z);
     stage_finish<S, C, STAGE, NTS>(z, Umu, Un, Xa, st, mu, s, live, eps);
 }
 
@@ -386,15 +414,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 // misses, so this form also shares the backward links U_nu(x - nu), which the
 // lower terms of three links of a site read, and keeps the workgroup's LDS
 // within a quarter of a CU's (four workgroups resident):
-//   - the plaquette projections as eight reals: the third imaginary diagonal
-//     entry is minus the sum of the other two (traceless);
+//   - the plaquettes folded and made traceless, as eight reals: the third
+//     imaginary diagonal entry is minus the sum of the other two;
 //   - rows 0 and 1 of the site's own links and of its backward links; row 2
 //     is rebuilt as the conjugated cross product where a whole matrix is
 //     needed (exact for SU(3), a rounding-level difference for the FP32
 //     links between projections).
 // 236 -> 180 link loads per site, about 110 more VALU instructions per wave.
 struct CompactLds {
-    float p[6][8][kTileSites];     // -1/2 TA(P_ab(x)): components 0, 1, 3..8 of the nine-real form
+    float p[6][8][kTileSites];     // P_ab(x) folded and traceless: components 0, 1, 3..8
     vf4 loc[4][3][kTileSites];     // rows 0, 1 of U_d(x)
     vf4 back[4][3][kTileSites];    // rows 0, 1 of U_d(x - d)
 };
@@ -429,14 +457,16 @@ LGF_DEV float plaquette_compact(CompactLds& L, const S* __restrict__ U, int64_t 
     const int fa = nb_off(g, A, +1), fb = nb_off(g, B, +1);
     const M3<float> h1 = su3_nn(rows01_load<false>(L.loc[A], lane), load_link<S, float>(U, st, B, s + fa));
     const M3<float> h2 = su3_nn(rows01_load<false>(L.loc[B], lane), load_link<S, float>(U, st, A, s + fb));
-    float t[9];
-    ta_pack(ta_of_product_nd(h1, h2, -0.5f), t);
+    Ta9<float> w;
+    set_zero(w);
+    mul_nd_acc_ta(w, h1, h2);
+    make_traceless(w);
     constexpr int p = plane_index(A, B);
-    L.p[p][0][lane] = t[0];
-    L.p[p][1][lane] = t[1];
+    L.p[p][0][lane] = w.v[0];
+    L.p[p][1][lane] = w.v[1];
 #pragma unroll
-    for (int k = 3; k < 9; ++k) L.p[p][k - 1][lane] = t[k];
-    return t[8];
+    for (int k = 3; k < 9; ++k) L.p[p][k - 1][lane] = w.v[k];
+    return w.v[8];
 }
 
 // lower term of the (MU,NU) plane with the backward link from LDS
@@ -476,7 +506,12 @@ LGF_DEV void stage_pre_pb(CompactLds& L, const S* __restrict__ U, int64_t st, Ge
     after<true>(g, A.v[17]);
     lower_term_compact<S, MU, N2>(A, L, U, st, g, fmu, lane);
     Umu = rows01_load<true>(L.loc[MU], lane);
-    ta_pack(ta_of_product(Umu, A, -0.5f), z);
+    Ta9<float> w;
+    set_zero(w);
+    mul_nn_acc_ta(w, Umu, A);
+    make_traceless(w);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) z[k] = w.v[k];
 }
 
 template <typename S, typename C, int STAGE, int WPE, bool NTS>
@@ -524,6 +559,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
         for (int c = 3; c < 9; ++c) z[c] += sg * L.p[p][c - 1][lane];
     }
     z[2] = -(z[0] + z[1]);  // the sum is traceless
+    minus_half_ta(z);
     stage_finish<S, C, STAGE, NTS>(z, Umu, Un, Xa, st, mu, s, live, eps);
 }
 

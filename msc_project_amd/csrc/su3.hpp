@@ -264,6 +264,17 @@ LGF_DEV void scale(Ta9<R>& t, R w) {
         o.v[4 + 2 * p] = y;                                                        \
     }
 
+// The folded sum made traceless: v[d] -= (v[0] + v[1] + v[2]) / 3.  The
+// nine reals of -1/2 TA(W) (ta_pack of ta_scaled(W, -1/2)) are then
+// (-v[0], -v[1], -v[2], -v[3]/2, ..., -v[8]/2), so sums of such
+// projections can be added in this form and scaled once.
+template <typename R>
+LGF_DEV void make_traceless(Ta9<R>& t) {
+    const R m = (t.v[0] + t.v[1] + t.v[2]) * (R(1) / R(3));
+#pragma unroll
+    for (int d = 0; d < 3; ++d) t.v[d] -= m;
+}
+
 // o += a b, a b^dag, a^dag b in the folded form
 template <typename R>
 LGF_DEV void mul_nn_acc_ta(Ta9<R>& o, const M3<R>& a, const M3<R>& b) {
