@@ -165,25 +165,41 @@ def test_config3_shapes_batched_as_alone(ctx):
 
 def test_fp32_tracks_fp64_on_thermalised_24(ctx):
     """the mixed-precision flow against the FP64 flow of the same thermalised
-    24^4 lattice (beta 6.0): |dQ| < 1e-4 and relative |d(t^2 E)| < 1e-6 at
-    every measured flow time, links unitary to 1e-6"""
+    24^4 lattice (beta 6.0) to t = 4, past t0 (t^2 E = 0.3): |dQ| < 1e-4 and
+    relative |d(t^2 E)| < 1e-6 at every measured flow time (every 0.2) and at
+    t0, interpolated linearly in t^2 E between the bracketing measurements;
+    links unitary to 1e-6"""
     g = lgf.Field(ctx, L24, lgf.FP64)
     g.set_cold()
-    g.heatbath(6.0, 77, 30)
+    g.heatbath(6.0, 77, 100)
     f64 = lgf.Field(ctx, L24, lgf.FP64)
     f64.copy_from(g)
     f32 = lgf.Field(ctx, L24, lgf.FP32)
     f32.copy_from(g)
     g.close()
-    t = 0.0
-    for k in range(6):
+    ts, e64, e32, q64, q32 = [], [], [], [], []
+    for k in range(21):
+        t = 0.2 * k
         a, b = f64.observables(), f32.observables()
-        assert abs(b["Q"] - a["Q"]) < 1e-4, (t, a["Q"], b["Q"])
-        if t > 0:
-            assert abs(b["E"] - a["E"]) / abs(a["E"]) < 1e-6, (t, a["E"], b["E"])
-        f64.flow(0.01, 20)
-        f32.flow(0.01, 20)
-        t += 0.2
+        ts.append(t), e64.append(a["E"]), e32.append(b["E"]), q64.append(a["Q"]), q32.append(b["Q"])
+        if k < 20:
+            f64.flow(0.01, 20)
+            f32.flow(0.01, 20)
+    ts, e64, e32, q64, q32 = map(np.array, (ts, e64, e32, q64, q32))
+    dq = np.abs(q32 - q64)
+    rel = np.abs(e32[1:] - e64[1:]) / np.abs(e64[1:])  # = relative |d(t^2 E)| for t > 0
+    print("max |dQ| %.2e, max rel d(t^2E) %.2e" % (dq.max(), rel.max()))
+    assert dq.max() < 1e-4
+    assert rel.max() < 1e-6
+    y64, y32 = ts ** 2 * e64, ts ** 2 * e32
+    i = int(np.argmax(y64 >= 0.3))
+    assert i > 0 and y64[i] >= 0.3, "t0 is inside the flow"
+    w = (0.3 - y64[i - 1]) / (y64[i] - y64[i - 1])
+    t0 = ts[i - 1] + w * (ts[i] - ts[i - 1])
+    y32_t0 = y32[i - 1] + w * (y32[i] - y32[i - 1])
+    dq_t0 = abs((q32[i - 1] - q64[i - 1]) * (1 - w) + (q32[i] - q64[i]) * w)
+    print("t0 %.4f: rel d(t^2E) %.2e, |dQ| %.2e" % (t0, abs(y32_t0 - 0.3) / 0.3, dq_t0))
+    assert abs(y32_t0 - 0.3) / 0.3 < 1e-6 and dq_t0 < 1e-4
     _, umax, _ = f32.diagnostics()
     assert umax < 1e-6
     f64.close()
