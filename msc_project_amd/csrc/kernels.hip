@@ -642,20 +642,22 @@ struct Lines {
 // occupancy; the cache shares its LDS with the per-plane field strengths,
 // which are written after every wave is done with it.  Without it the
 // sweep is 9 % slower (24^4).
+//   LGF_CLOVER_XLINES_ROWS: the same cache for the FP32 sweep, holding rows
+//   0 and 1 of each line (37 KB, within the LDS that sweep already uses for
+//   its field strengths, so it costs no residency): that sweep is bound by
+//   the bytes its loads return through the vector memory path, and the
+//   cache makes it 2-3 % faster (24^4).  The FP64-arithmetic sweep over FP32
+//   links does not use it: with it that kernel needs fewer registers and two
+//   workgroups share a CU, which thrashes its caches (33 % slower), and at
+//   its old residency (LDS padded) it is no faster.
 #ifndef LGF_CLOVER_XLINES
 #define LGF_CLOVER_XLINES 1
 #endif
-template <typename S>
-constexpr bool kCloverXLines = LGF_CLOVER_XLINES && sizeof(S) == 8;
-
-template <int LEN, typename S, typename C>
-LGF_DEV M3<C> xline(const Lines<S, C>& L, int dir) {
-    M3<C> m;
-    const C* q = L.xl + (size_t)(4 * (LEN - 1) + dir) * 18 * kTileSites + L.lane;
-#pragma unroll
-    for (int c = 0; c < 18; ++c) m.v[c] = q[c * kTileSites];
-    return m;
-}
+#ifndef LGF_CLOVER_XLINES_ROWS
+#define LGF_CLOVER_XLINES_ROWS 1
+#endif
+template <typename S, typename C>
+constexpr bool kCloverXLines = sizeof(S) == 8 ? LGF_CLOVER_XLINES : sizeof(C) == 4 && LGF_CLOVER_XLINES_ROWS;
 
 // Products of lines.  With FP32 links (the mixed-precision path, whose links
 // are projected onto SU(3) every flow step) they are SU(3) products: rows 0
@@ -676,6 +678,22 @@ template <typename S>
 constexpr bool kCloverSu3 = sizeof(S) == 4;
 template <typename S, typename C>
 constexpr bool kCloverRows = LGF_CLOVER_ROWS2 && kCloverSu3<S>;
+// reals per line in the LDS line cache
+template <typename S, typename C>
+constexpr int kXLen = kCloverRows<S, C> ? 12 : 18;
+
+// a line that starts at the workgroup's own site x, from the LDS cache:
+// whole, or (ROWS = 2) rows 0 and 1 where the cache holds only those
+template <int LEN, int ROWS, typename S, typename C>
+LGF_DEV M3<C> xline(const Lines<S, C>& L, int dir) {
+    constexpr int X = kXLen<S, C>;
+    M3<C> m;
+    const C* q = L.xl + (size_t)(4 * (LEN - 1) + dir) * X * kTileSites + L.lane;
+#pragma unroll
+    for (int c = 0; c < X; ++c) m.v[c] = q[c * kTileSites];
+    if constexpr (X == 12 && ROWS != 2) su3_third_row(m);
+    return m;
+}
 
 // ROWS = 2: rows 0, 1; 3: whole; 4: columns 0, 1 (a left factor of
 // mul_dd_r01 or a right factor of mul_nn_c01), which for whole FP32 lines
@@ -736,7 +754,7 @@ struct PlaneOff {
 // a whole line that starts at the workgroup's own site x
 template <int LEN, typename S, typename C>
 LGF_DEV M3<C> own_line(const Lines<S, C>& L, int dir, int s) {
-    if constexpr (kCloverXLines<S>) return xline<LEN>(L, dir);
+    if constexpr (kCloverXLines<S, C>) return xline<LEN, 3>(L, dir);
     else return line<LEN, 3>(L, dir, s);
 }
 
@@ -750,7 +768,7 @@ LGF_DEV void leaves_a(M3<C>& Y, C w, bool first, C dep, const Lines<S, C>& L, in
     M3<C> p1, p4;
     if constexpr (kCloverSu3<S>) {
         const M3<C> mid1 = mul_nd_r01(line<M, 2>(L, mu, s + o.n[3 + N]), line<N, 3>(L, nu, s + o.m[3 + M]));
-        p1 = mul_nd_su3(mid1, line<M, 3>(L, mu, s));
+        p1 = mul_nd_su3(mid1, own_line<M>(L, mu, s));
     } else {
         const M3<C> mid1 = mul_nd(line<M, 3>(L, mu, s + o.n[3 + N]), line<N, 3>(L, nu, s + o.m[3 + M]));
         p1 = mul_nd(mid1, own_line<M>(L, mu, s));
@@ -782,7 +800,7 @@ LGF_DEV void leaves_b(M3<C>& Z, C w, bool first, C dep, const Lines<S, C>& L, in
     const int sn = s + o.n[3 - N];  // x - n nu
     if constexpr (kCloverSu3<S>) {
         const M3<C> h3 = mul_nn_r01(line<M, 2>(L, mu, sn), line<N, 3>(L, nu, sn + o.m[3 + M]));
-        const M3<C> t3d = mul_nd_su3(h3, line<M, 3>(L, mu, s));  // T3^dag
+        const M3<C> t3d = mul_nd_su3(h3, own_line<M>(L, mu, s));  // T3^dag
         if (first) set_scaled_dag(Z, w, t3d);
         else axpy_dag(Z, w, t3d);
     } else {
@@ -976,7 +994,7 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     int64_t G, double* __restrict__ partials) {
     // F values are computed in C; keeping them as C in LDS loses nothing
     using T = typename std::conditional<LGF_CLOVER_LDS_REAL != 0, C, double>::type;
-    constexpr bool XL = kCloverXLines<S>;
+    constexpr bool XL = kCloverXLines<S, C>;
     struct Out {
         T F[6][9][kTileSites];
         T F11[6][9][kTileSites];
@@ -985,7 +1003,7 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     };
     union Shared {
         Out o;
-        C xl[XL ? 12 : 1][18][kTileSites];
+        C xl[XL ? 12 : 1][kXLen<S, C>][kTileSites];
     };
     __shared__ Shared sh;
     T (*sF)[9][kTileSites] = sh.o.F;
@@ -1013,10 +1031,15 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int k = 2 * p + h, dir = k & 3;
-                const M3<C> m = k < 4 ? load_link<S, C>(L.U, L.st, dir, s)
-                                      : load_link<C, C>(k < 8 ? L.L2 : L.L3, G, dir, L.lbase + s);
+                M3<C> m;
+                if constexpr (kCloverRows<S, C>)
+                    m = k < 4 ? load_link_rows01<S, C>(L.U, L.st, dir, s)
+                              : load_link_rows01<C, C>(k < 8 ? L.L2 : L.L3, G, dir, L.lbase + s);
+                else
+                    m = k < 4 ? load_link<S, C>(L.U, L.st, dir, s)
+                              : load_link<C, C>(k < 8 ? L.L2 : L.L3, G, dir, L.lbase + s);
 #pragma unroll
-                for (int c = 0; c < 18; ++c) sh.xl[k][c][threadIdx.x] = m.v[c];
+                for (int c = 0; c < kXLen<S, C>; ++c) sh.xl[k][c][threadIdx.x] = m.v[c];
             }
         }
         __syncthreads();
