@@ -176,6 +176,33 @@ def accuracy_run(lgf, ctx, src, prec, steps=ACCURACY_STEPS):
             "unitarity_max": float(unit[:, 0].max())}
 
 
+def reference_order_fp64_rate(lgf, ctx, src, ev):
+    """Flow-only rate of the FP64 per-link stage kernel, which sums each
+    link's staples in the reference's order (tuned within its variant group,
+    graph-replayed steps, HIP events on the library's stream).  The context's
+    FP64 variant and tile order are restored afterwards."""
+    keep_v, keep_o = ctx.stage_variant(lgf.FP64), ctx.tile_order()
+    ctx.set_stage_variant(lgf.FP64, lgf.FP64_SHARED_PLAQUETTE_VARIANTS)
+    variant, _ = ctx.autotune(SHAPE, lgf.FP64, 10)
+    order = ctx.tile_order()
+    f = lgf.Field(ctx, SHAPE, lgf.FP64)
+    f.copy_from(src)
+    b = lgf.Batch(ctx, [f])
+    b.flow(EPS, 10, use_graph=True)
+    stream = ctx.stream()
+    e0, e1 = ev.event(), ev.event()
+    ev.record(e0, stream)
+    b.flow(EPS, FLOW_TIMING_STEPS, use_graph=True)
+    ev.record(e1, stream)
+    ms = ev.elapsed_ms(e0, e1) / FLOW_TIMING_STEPS
+    b.close()
+    f.close()
+    ctx.set_stage_variant(lgf.FP64, keep_v)
+    ctx.set_tile_order(keep_o)
+    return {"flow_only_lups": 4.0 * np.prod(SHAPE) / (ms * 1e-3), "flow_ms_per_step": ms, "stage_variant": variant,
+            "tile_order": order}
+
+
 def t0_of(t, E):
     from msc_project_amd import lgf
     return lgf.flow_scale_t0(t, E, 0.3)
@@ -520,6 +547,11 @@ def main():
         log("%s: |U^dag U - 1| max/mean at t = 0, 0.05, ..., 5: %s" % (
             name, " ".join("%.2e/%.2e" % (a, m) for a, m in accr[prec]["unitarity"])))
     lu_step = 4.0 * np.prod(SHAPE)
+    # beside the FP64 kernel the autotuner chose (shared plaquettes), the
+    # per-link FP64 kernel that sums the staples in the reference's order
+    ref64 = reference_order_fp64_rate(lgf, ctx, src, ev)
+    log("fp64, per-link kernel (reference staple order): flow %.4f ms/step, variant %d, tile order %d" % (
+        ref64["flow_ms_per_step"], ref64["stage_variant"], ref64["tile_order"]))
     r32, r64 = res[lgf.FP32], res[lgf.FP64]
     a32, a64 = accr[lgf.FP32], accr[lgf.FP64]
     acc = accuracy(a64, a32)
@@ -559,6 +591,8 @@ def main():
         "fp32_flow_only_lups": lu_step / (r32["flow_ms_per_step"] * 1e-3),
         "fp64_flow_only_lups": lu_step / (r64["flow_ms_per_step"] * 1e-3),
         "fp32_over_fp64_flow": r64["flow_ms_per_step"] / r32["flow_ms_per_step"],
+        "fp64_reference_order": dict(ref64, fp32_over=ref64["flow_ms_per_step"] / r32["flow_ms_per_step"],
+                                     kernel="k_rk3_stage<double> (per link, the reference's staple order)"),
         "measure_ms": {"fp32": r32["measure_ms"], "fp64": r64["measure_ms"]},
         "fp32_vs_fp64": acc,
         "unitarity_max": {"fp32": a32["unitarity_max"], "fp64": a64["unitarity_max"]},
