@@ -1,5 +1,5 @@
-"""Build check of the compiled RK3 stage kernels: every global memory access
-must take its address from a pointer.
+"""Build check of the compiled RK3 stage, line-product and clover kernels:
+every global memory access must take its address from a pointer.
 
 A shared-backward-link stage variant once faulted the GPU because the
 compiler merged the tails of the four direction cases of a `switch` into one
@@ -10,7 +10,8 @@ the gfx950 disassembly of the kernels built here (the code object embedded
 in _build/kernels.hip.o) and replays that path-by-path reading for every
 stage kernel and every direction a wave can take:
 
-- the direction (threadIdx.y or blockIdx.y, 0..3) is fixed per run, and
+- the direction (threadIdx.y or blockIdx.y: 0..3 for the stage and
+  line-product kernels, the plane 0..5 for the clover) is fixed per run, and
   scalar constants, SCC, VCC and EXEC (zero / non-zero) are propagated, so
   that branches on the direction are decided and only the paths that a wave
   of that direction can take are followed (conditional constant
@@ -26,8 +27,11 @@ stage kernel and every direction a wave can take:
     python check_load_bases.py [_build/kernels.hip.o]     exit 1 on findings
 
 The stage kernels take (FieldDesc* fds, int2* tiles, int ping): kernel
-argument bytes 0..15 are the two pointers, and FieldDesc's pointer fields
-(links[0], links[1], acc, qx) are its first 32 bytes.
+argument bytes 0..15 are the two pointers.  The line-product kernel takes
+(fds, tiles, int t0, int ping, C* lines, int64_t G) and the clover kernel
+(fds, tiles, t0, ping, lines, G, double* partials): pointers at bytes 0, 8,
+24 and 40.  FieldDesc's pointer fields (links[0], links[1], acc, qx) are its
+first 32 bytes.
 """
 import concurrent.futures
 import os
@@ -39,8 +43,15 @@ import tempfile
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LLVM = "/opt/rocm/llvm/bin"
-KERNEL_RE = re.compile(r"_ZN3lgf1[0-9]k_rk3_stage")  # k_rk3_stage, k_rk3_stage_pl, k_rk3_stage_pb
-DIRECTIONS = 4
+# (name pattern, directions a wave can take, byte offsets of the kernel's pointer arguments)
+FAMILIES = (
+    (re.compile(r"_ZN3lgf1[0-9]k_rk3_stage"), 4, (0, 8)),  # k_rk3_stage, k_rk3_stage_pl, k_rk3_stage_pb
+    (re.compile(r"_ZN3lgf7k_lines"), 4, (0, 8, 24)),
+    (re.compile(r"_ZN3lgf8k_clover"), 6, (0, 8, 24, 40)),
+)
+STAGE_FAMILY = FAMILIES[0][1:]
+# dword offsets of the pointer arguments of the kernel being checked (set by check_kernel)
+pointer_dwords = frozenset([0, 4, 8, 12])
 MAX_VALUES = 48  # distinct provenances kept per register before collapsing to "unknown"
 
 UNKNOWN = frozenset([("unknown",)])  # the analysis gave up: taken as a possible pointer
@@ -49,6 +60,7 @@ MEM = frozenset([("mem",)])  # data loaded by a vector memory instruction
 TID = frozenset([("tid",)])  # packed work-item id (v0 at entry)
 TIDY = frozenset([("tidy",)])  # threadIdx.y
 IMM = frozenset([("imm",)])  # made from constants only, not evaluated
+VCC_HALVES = ("vcc_lo", "vcc_hi")  # used as plain SGPRs in address arithmetic as well as the VCC mask
 Z, NZ, U, ONES = "zero", "nonzero", "unknown", "ones"  # EXEC / VCC / mask states
 M32 = 0xFFFFFFFF
 
@@ -104,7 +116,7 @@ def parse(text):
         m = re.match(r"^([0-9a-f]+) <(\S+)>:$", line)
         if m:
             name, start = m.group(2), int(m.group(1), 16)
-            cur = kernels.setdefault(name, []) if KERNEL_RE.match(name) else None
+            cur = kernels.setdefault(name, []) if family(name) else None
             continue
         if cur is None or not line.startswith("\t"):
             continue
@@ -184,9 +196,18 @@ def load_atom(base_values, off):
     return ("other", off)
 
 
+def family(name):
+    """(directions, pointer argument offsets) of a kernel name, or None"""
+    for pat, n, ptrs in FAMILIES:
+        if pat.match(name):
+            return n, ptrs
+    return None
+
+
 def is_pointer(atom):
-    """both dwords of the two pointer arguments and of the descriptor's four pointer fields"""
-    return atom == ("unknown",) or (atom[0] == "arg" and atom[1] < 16) or (atom[0] == "desc" and atom[1] < 32)
+    """both dwords of the kernel's pointer arguments and of the descriptor's four pointer fields"""
+    return atom == ("unknown",) or (atom[0] == "arg" and atom[1] in pointer_dwords) or \
+        (atom[0] == "desc" and atom[1] < 32)
 
 
 def signed(x):
@@ -251,7 +272,7 @@ class State:
     def get(self, k):
         v = self.r.get(k)
         if v is None:
-            return {frozenset([("entry", str(k))])}
+            return {LANE} if k in VCC_HALVES else {frozenset([("entry", str(k))])}
         return v
 
     def val(self, tok):
@@ -259,10 +280,14 @@ class State:
         rr = regs(tok)
         if rr:
             return self.get(rr[0]) if len(rr) == 1 else set().union(*[self.get(k) for k in rr])
+        if tok in VCC_HALVES:
+            return self.get(tok)
+        if tok == "vcc":
+            return self.get(VCC_HALVES[0]) | self.get(VCC_HALVES[1])
         c = imm(tok)
         if c is not None:
             return const(c)
-        return {LANE}  # vcc_lo, float literal, exec, m0 ...
+        return {LANE}  # float literal, exec, m0 ...
 
     def join(self, o):
         """merge o into self; True if anything changed"""
@@ -419,13 +444,18 @@ def scalar(st, mn, ops):
             if not keep_scc:
                 st.scc = None
         setattr(st, "exec" if d == "exec" else "vcc", res)
+        if d == "vcc":
+            halves = operand_dwords(st, srcs[0], 2) if mn == "s_mov_b64" else [{LANE}] * 2
+            st.r.update(zip(VCC_HALVES, halves))
         return
-    if d.startswith(("exec_", "vcc_")):
-        setattr(st, "exec" if d.startswith("exec") else "vcc", U)
+    if d.startswith("exec_"):
+        st.exec = U
         if not keep_scc:
             st.scc = None
         return
-    dst = sreg_range(d)
+    if d in VCC_HALVES:
+        st.vcc = U
+    dst = [d] if d in VCC_HALVES else sreg_range(d)
     if dst is None:
         if not keep_scc:
             st.scc = None
@@ -450,7 +480,7 @@ def scalar(st, mn, ops):
             if not keep_scc:
                 st.scc, keep_scc = r[1], True
     if new is None:
-        v = combine([st.val(s) for s in srcs if regs(s)])
+        v = combine([st.val(s) for s in srcs if regs(s) or s in VCC_HALVES + ("vcc",)])
         new = [v] * len(dst)
     for r, v in zip(dst, new):
         st.r[r] = set(v)
@@ -460,13 +490,19 @@ def scalar(st, mn, ops):
 
 def operand_dwords(st, tok, n):
     """per-dword provenance of a scalar source operand of n dwords"""
-    rr = sreg_range(tok)
+    rr = list(VCC_HALVES) if tok == "vcc" else sreg_range(tok)
     if rr and len(rr) == n:
         return [set(st.get(r)) for r in rr]
     c = imm(tok)
     if c is not None:
         return const64(c) if n == 2 else [const(c)]
     return [st.val(tok)] * n
+
+
+def vcc_mask(st):
+    """VCC written as a lane mask by a vector instruction"""
+    st.vcc = U
+    st.r.update((h, {LANE}) for h in VCC_HALVES)
 
 
 def vector(st, mn, ops, direction):
@@ -478,17 +514,17 @@ def vector(st, mn, ops, direction):
         return
     if mn.startswith("v_cmp"):
         if ops[0] == "vcc" or mn.endswith("_e32"):
-            st.vcc = U
+            vcc_mask(st)
         for r in sreg_range(ops[0]) or []:
             st.r[r] = {LANE}
         return
     srcs = ops[1:]
     if "_co_" in mn or mn.startswith(("v_addc", "v_subb", "v_div_scale", "v_mad_u64", "v_mad_i64")):
         if mn.endswith("_e32") and "_co_" in mn:
-            st.vcc = U
+            vcc_mask(st)
         elif len(ops) > 1 and (ops[1] == "vcc" or sreg_range(ops[1])):
             if ops[1] == "vcc":
-                st.vcc = U
+                vcc_mask(st)
             for r in sreg_range(ops[1]) or []:
                 st.r[r] = {LANE}
             srcs = ops[2:]
@@ -514,7 +550,9 @@ def vector(st, mn, ops, direction):
         for r in sreg_range(d) or []:  # VOP3 forms with a scalar destination
             st.r[r] = {LANE}
         if d == "vcc":
-            st.vcc = U
+            vcc_mask(st)
+        elif d in VCC_HALVES:
+            st.vcc, st.r[d] = U, {LANE}
         return
     if mn.startswith(("v_mov_b32", "v_mov_b64")):
         rr = regs(srcs[0])
@@ -529,7 +567,7 @@ def vector(st, mn, ops, direction):
     elif mn.startswith("v_bfe_u32") and st.val(srcs[0]) == {TID} and imm(srcs[1]) == 10 and imm(srcs[2]) == 10:
         vals = [{TIDY}] * len(dst)
     else:
-        sv = [st.val(s) for s in srcs if regs(s)]
+        sv = [st.val(s) for s in srcs if regs(s) or s in VCC_HALVES + ("vcc",)]
         plain = all(a[0] in PLAIN_ATOMS for s in sv for v in s for a in v)
         vals = [{LANE} if plain else combine(sv)] * len(dst)
     for r, v in zip(dst, vals):
@@ -580,12 +618,16 @@ def successors(st, ins, fall, tb):
     return [(tb, st.copy()), (fall, st)]
 
 
-def check_kernel(instrs, setup):
+def check_kernel(instrs, setup, fam=STAGE_FAMILY):
     """(sorted findings over every direction: (address, text, provenance, direction),
-    number of global accesses checked on some path with active lanes)"""
+    number of global accesses checked on some path with active lanes); fam =
+    (directions, pointer argument offsets), the stage kernels' by default"""
+    global pointer_dwords
+    directions, ptrs = fam
+    pointer_dwords = frozenset(o + k for o in ptrs for k in (0, 4))
     blocks, bid = blocks_of(instrs)
     found, checked = set(), set()
-    for direction in range(DIRECTIONS):
+    for direction in range(directions):
         ins_state = [dict() for _ in blocks]  # block -> {EXEC state: State}
         st0 = entry_state(setup, direction)
         ins_state[0][st0.exec] = st0
@@ -660,12 +702,13 @@ def disassemble(obj):
 
 
 def check_object(obj, workers=8):
-    """{kernel: (findings, accesses checked)} for every stage kernel in a host object"""
+    """{kernel: (findings, accesses checked)} for every checked kernel in a host object"""
     text, setup = disassemble(obj)
     kernels = parse(text)
     names = sorted(kernels)
     with concurrent.futures.ProcessPoolExecutor(max_workers=workers) as pool:
-        results = pool.map(check_kernel, [kernels[n] for n in names], [setup[n] for n in names])
+        results = pool.map(check_kernel, [kernels[n] for n in names], [setup[n] for n in names],
+                           [family(n) for n in names])
         return dict(zip(names, results))
 
 
@@ -678,8 +721,8 @@ def main():
         for addr, text, prov, direction in findings:
             bad += 1
             print("%s direction %d @%x: %s: address may hold %s" % (name, direction, addr, text, list(prov)))
-    print("check_load_bases: %d stage kernels, %d global accesses checked, %d through a non-pointer address"
-          % (len(results), checked, bad))
+    print("check_load_bases: %d kernels (RK3 stage, line products, clover), %d global accesses checked, "
+          "%d through a non-pointer address" % (len(results), checked, bad))
     return 1 if bad else 0
 
 

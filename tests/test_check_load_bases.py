@@ -1,8 +1,8 @@
-"""The build check of the stage kernels' global-access addresses
-(msc_project_amd/check_load_bases.py): planted defects in small hand-written
-gfx950 listings are found on exactly the direction whose path carries them,
-paths that no wave can take are not reported, and the compiled library's
-stage kernels pass."""
+"""The build check of the stage, line-product and clover kernels'
+global-access addresses (msc_project_amd/check_load_bases.py): planted
+defects in small hand-written gfx950 listings are found on exactly the
+direction whose path carries them, paths that no wave can take are not
+reported, and the compiled library's kernels pass."""
 import os
 import sys
 
@@ -44,9 +44,9 @@ def listing(lines, name=NAME):
     return "\n".join(out) + "\n"
 
 
-def findings(lines, setup=TIDY_SETUP):
-    (instrs,) = clb.parse(listing(lines)).values()
-    found, checked = clb.check_kernel(instrs, setup)
+def findings(lines, setup=TIDY_SETUP, name=NAME):
+    (instrs,) = clb.parse(listing(lines, name)).values()
+    found, checked = clb.check_kernel(instrs, setup, clb.family(name))
     return found, checked
 
 
@@ -178,6 +178,49 @@ def test_vector_addresses_and_lane_spills():
     assert {f[0] for f in found} == {BASE + 8 * (len(PROLOGUE) + 9)}
 
 
+def test_vcc_halves_as_address_registers():
+    # the compiler also uses vcc_lo / vcc_hi as two plain SGPRs in address arithmetic
+    lines = PROLOGUE + [
+        "s_load_dwordx2 s[20:21], s[4:5], 0x38",  # a stride field of the descriptor
+        "s_waitcnt lgkmcnt(0)",
+        "s_add_u32 vcc_lo, s8, s20",  # links[0] + stride
+        "s_addc_u32 vcc_hi, s9, s21",
+        "s_add_u32 s12, vcc_lo, s20",
+        "s_addc_u32 s13, vcc_hi, s21",
+        "global_load_dwordx4 v[2:5], v6, s[12:13]",
+        "v_cmp_gt_u32_e32 vcc, 4, v6",  # VCC rewritten as a lane mask
+        "s_add_u32 s14, vcc_lo, s20",
+        "s_addc_u32 s15, vcc_hi, s21",
+        "global_load_dwordx4 v[2:5], v6, s[14:15]",
+        "s_endpgm",
+    ]
+    found, checked = findings(lines)
+    assert checked == 2
+    assert {f[0] for f in found} == {BASE + 8 * (len(PROLOGUE) + 10)}
+
+
+def test_pointer_arguments_of_the_clover_kernel():
+    # k_clover(fds, tiles, int t0, int ping, lines, int64_t G, partials): bytes 24 and 40 are pointers, 16 and 32 not
+    name = "_ZN3lgf8k_clovertest"
+    lines = PROLOGUE + [
+        "s_load_dwordx2 s[10:11], s[0:1], 0x18",  # lines
+        "s_load_dwordx2 s[12:13], s[0:1], 0x28",  # partials
+        "s_load_dwordx2 s[14:15], s[0:1], 0x20",  # G
+        "s_waitcnt lgkmcnt(0)",
+        "global_load_dwordx4 v[2:5], v6, s[10:11]",
+        "global_store_dwordx2 v6, v[2:3], s[12:13]",
+        "s_cmp_eq_u32 s6, 5",  # only plane 5 adds G to a pointer by mistake
+        "s_cselect_b64 s[16:17], s[14:15], s[10:11]",
+        "global_load_dwordx4 v[2:5], v6, s[16:17]",
+        "s_endpgm",
+    ]
+    found, checked = findings(lines, name=name)
+    assert checked == 3
+    assert {f[0] for f in found} == {BASE + 8 * (len(PROLOGUE) + 8)} and {f[3] for f in found} == {5}
+    assert clb.family(name) == (6, (0, 8, 24, 40))
+    assert clb.family("_ZN3lgf7k_linesItest") == (4, (0, 8, 24))
+
+
 def test_elided_padding_lines_are_skipped():
     text = listing(PROLOGUE + ["global_load_dwordx4 v[2:5], v6, s[8:9]", "s_endpgm"]) + "\t\t...\n"
     (instrs,) = clb.parse(text).values()
@@ -188,9 +231,11 @@ OBJ = os.path.join(ROOT, "msc_project_amd", "_build", "kernels.hip.o")
 
 
 @pytest.mark.skipif(not os.path.exists(OBJ), reason="the kernel object is built by __graft_entry__.build()")
-def test_compiled_stage_kernels_pass():
+def test_compiled_kernels_pass():
     results = clb.check_object(OBJ)
     assert len(results) >= 40
+    assert any(k.startswith("_ZN3lgf7k_lines") for k in results)
+    assert any(k.startswith("_ZN3lgf8k_clover") for k in results)
     assert sum(n for _, n in results.values()) > 10000
     bad = [(k, f) for k, (fs, _) in results.items() for f in fs]
     assert not bad, bad[:5]
