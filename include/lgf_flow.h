@@ -131,7 +131,9 @@ int lgf_ctx_get_batch_group_bytes(lgf_ctx* ctx, size_t* bytes);
  * that fit `bytes`, a single larger field on its own.  Default 1 GiB.  The
  * scratch is kept by the context until it is destroyed. */
 int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes);
-/* time every pair of tile order (0, 2, 4, 8, 404, 408) and stage variant for
+/* tile orders the autotuners time (lgf_ctx_set_tile_order codes) */
+#define LGF_AUTOTUNE_TILE_ORDERS {0, 2, 4, 8, 404, 408, 604, 1204, 6402, 6404}
+/* time every pair of tile order (LGF_AUTOTUNE_TILE_ORDERS) and stage variant for
  * nsteps RK3 steps on a seeded lattice of `shape` (each pair three times, its
  * fastest run counting), and make the fastest pair this context's tile order
  * and its stage variant for `precision`; ms_per_step (optional) gets one

@@ -969,7 +969,7 @@ static bool tunable(int precision, int current, int v) {
 
 static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int precision, int nsteps, int* best,
                            double* ms_per_step) {
-    static const int orders[] = {0, 2, 4, 8, 404, 408};
+    static const int orders[] = LGF_AUTOTUNE_TILE_ORDERS;
     constexpr int norders = sizeof(orders) / sizeof(orders[0]);
     const int nv = num_stage_variants(precision);
     const int keep_order = ctx->tile_block, keep_variant = ctx->stage_variant[precision];

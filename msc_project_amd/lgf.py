@@ -31,6 +31,8 @@ FP32_COMPACT_VARIANTS = 6
 # others sum each link's staples in the reference's order
 # (LGF_FP64_SHARED_PLAQUETTE_VARIANTS)
 FP64_SHARED_PLAQUETTE_VARIANTS = 3
+# tile orders the autotuners time (LGF_AUTOTUNE_TILE_ORDERS)
+AUTOTUNE_TILE_ORDERS = (0, 2, 4, 8, 404, 408, 604, 1204, 6402, 6404)
 # variant ranges whose links agree bit for bit
 FP32_VARIANT_GROUPS = ((0, FP32_COMPACT_VARIANTS), (FP32_COMPACT_VARIANTS, FP32_SHARED_PLAQUETTE_VARIANTS),
                        (FP32_SHARED_PLAQUETTE_VARIANTS, None))

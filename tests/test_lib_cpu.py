@@ -24,6 +24,16 @@ def test_header_declares_api():
     assert set(names) == set(lgf.SIGNATURES), set(names) ^ set(lgf.SIGNATURES)
 
 
+def test_binding_constants_match_the_header():
+    src = open(os.path.join(ROOT, "include", "lgf_flow.h")).read()
+    macros = dict(re.findall(r"#define (LGF_[A-Z0-9_]+) (.+)", src))
+    assert lgf.FP32_SHARED_PLAQUETTE_VARIANTS == int(macros["LGF_FP32_SHARED_PLAQUETTE_VARIANTS"])
+    assert lgf.FP32_COMPACT_VARIANTS == int(macros["LGF_FP32_COMPACT_VARIANTS"])
+    assert lgf.FP64_SHARED_PLAQUETTE_VARIANTS == int(macros["LGF_FP64_SHARED_PLAQUETTE_VARIANTS"])
+    orders = tuple(int(v) for v in macros["LGF_AUTOTUNE_TILE_ORDERS"].strip(" {}").split(","))
+    assert lgf.AUTOTUNE_TILE_ORDERS == orders
+
+
 def test_library_exports_every_declared_symbol():
     L = ctypes.CDLL(lgf.LIB_PATH)
     for name in declared_functions():
