@@ -135,10 +135,12 @@ int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes);
 #define LGF_AUTOTUNE_TILE_ORDERS {0, 2, 4, 8, 404, 408, 604, 1204, 6402, 6404}
 /* time every pair of tile order (LGF_AUTOTUNE_TILE_ORDERS) and stage variant for
  * nsteps RK3 steps on a seeded lattice of `shape` (each pair three times, its
- * fastest run counting), and make the fastest pair this context's tile order
- * and its stage variant for `precision`; ms_per_step (optional) gets one
- * entry per variant, at the chosen order (HUGE_VAL for FP64 variants outside
- * the group of the context's variant, which are not timed).  No counterpart in the reference
+ * fastest run counting); time the four fastest pairs again with 3*nsteps
+ * steps, then every variant at the order of the fastest of those, and make
+ * that order and the fastest of those variants this context's tile order and
+ * stage variant for `precision`; ms_per_step (optional) gets those last
+ * times, one entry per variant (HUGE_VAL for FP64 variants outside the group
+ * of the context's variant, which are not timed).  No counterpart in the reference
  * (its flow has no launch parameters to choose). */
 int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, int* best, double* ms_per_step);
 /* the same search on `nshapes` seeded lattices (shapes[4*i .. 4*i+3])
@@ -147,8 +149,8 @@ int lgf_autotune(lgf_ctx* ctx, const int shape[4], int precision, int nsteps, in
  * lattice's.  With more than one lattice the pairs are timed with the
  * batch as one group and in cache groups of 224 MiB
  * (lgf_ctx_set_batch_group_bytes), and the cap is then chosen among 0, 128,
- * 192, 224 and 256 MiB at the best pair; ms_per_step is at the best cap of
- * the search.  The lattices of all shapes are held at once (the sum of their
+ * 192, 224 and 256 MiB at the chosen pair; ms_per_step is at the cap of the
+ * search that the chosen pair was timed with.  The lattices of all shapes are held at once (the sum of their
  * lgf_field_footprint) and released before it returns.  LGF_E_ARG for
  * nshapes <= 0, LGF_E_SHAPE for a bad shape; on failure the context's
  * tile order, variant and group cap are left as they were. */

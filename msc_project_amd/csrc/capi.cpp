@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -915,13 +916,20 @@ extern "C" int lgf_measure_finish(lgf_field* f, double* obs) {
 // Times every (tile order, stage-kernel variant) pair on seeded random
 // lattices of the given shapes, flowed together as one batch (nsteps RK3
 // steps per timing, after one warm-up step, HIP events on the context's
-// stream), and makes the fastest pair the context's tile order and variant
-// for that precision.  The pairs are searched jointly: the fastest order
+// stream), and makes the fastest pair (after the re-timing below) the
+// context's tile order and variant for that precision.  The pairs are searched jointly: the fastest order
 // depends on the variant (non-temporal stores change which neighbours stay
 // in cache).  ms_per_step (optional) receives one time per variant, at the
 // chosen order.
 // each candidate is timed kAutotuneReps times and its fastest run counts
 constexpr int kAutotuneReps = 3;
+// The search's trials are short (a pair's timings differ by a few per cent
+// from run to run, as much as the variants differ), so its kRefinePairs
+// fastest pairs are timed again with kRefineSteps times the steps, and then
+// every variant at the (cache cap, order) of the fastest of those; that
+// row's fastest variant is chosen.
+constexpr int kRefinePairs = 4;
+constexpr int kRefineSteps = 3;
 
 static int time_steps(lgf_ctx* ctx, lgf_batch* b, const double* eps, int nsteps, hipEvent_t e0, hipEvent_t e1,
                       float* ms) {
@@ -1006,22 +1014,47 @@ static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int pre
     }
     int bo = 0, bv = 0, bc = 0;
     size_t bg = keep_groups;
+    std::vector<double> row(nv, HUGE_VAL);  // every variant at the chosen cap and order, re-timed
+    const int long_steps = nsteps <= INT_MAX / kRefineSteps ? nsteps * kRefineSteps : nsteps;
     if (!rc) {
-        const size_t arg = size_t(std::min_element(ms.begin(), ms.end()) - ms.begin());
+        std::vector<size_t> idx(ms.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        const size_t k = std::min<size_t>(kRefinePairs, idx.size());
+        std::partial_sort(idx.begin(), idx.begin() + k, idx.end(), [&](size_t a, size_t b) { return ms[a] < ms[b]; });
+        size_t arg = idx[0];
+        double best_ms = HUGE_VAL;
+        for (size_t j = 0; j < k && !rc && ms[idx[j]] < HUGE_VAL; ++j) {
+            const size_t a = idx[j];
+            ctx->group_bytes = search_caps[a / (size_t(norders) * nv)];
+            ctx->tile_block = orders[a / nv % norders];
+            ctx->stage_variant[precision] = int(a % nv);
+            double m = 0;
+            rc = time_batch(ctx, fields, eps, long_steps, e0, e1, &m);
+            if (!rc && m < best_ms) {
+                best_ms = m;
+                arg = a;
+            }
+        }
         bc = int(arg / (size_t(norders) * nv));
         bo = int(arg / nv % norders);
-        bv = int(arg % nv);
+        ctx->group_bytes = search_caps[bc];
         ctx->tile_block = orders[bo];
+        for (int v = 0; v < nv && !rc; ++v) {
+            if (!tunable(precision, keep_variant, v)) continue;
+            ctx->stage_variant[precision] = v;
+            rc = time_batch(ctx, fields, eps, long_steps, e0, e1, &row[v]);
+        }
+        bv = int(std::min_element(row.begin(), row.end()) - row.begin());
         ctx->stage_variant[precision] = bv;
-        if (fields.size() > 1) {
-            // refine the cache-group cap at the best pair
-            double best_ms = ms[arg];
+        if (!rc && fields.size() > 1) {
+            // refine the cache-group cap at the chosen pair
+            best_ms = row[bv];
             bg = search_caps[bc];
             for (size_t cap : kGroupCaps) {
                 if (cap == search_caps[0] || cap == search_caps[1]) continue;
                 ctx->group_bytes = cap;
                 double m = 0;
-                rc = time_batch(ctx, fields, eps, nsteps, e0, e1, &m);
+                rc = time_batch(ctx, fields, eps, long_steps, e0, e1, &m);
                 if (rc) break;
                 if (m < best_ms) {
                     best_ms = m;
@@ -1044,7 +1077,7 @@ static int autotune_shapes(lgf_ctx* ctx, const int* shapes, int nshapes, int pre
     ctx->group_bytes = bg;
     if (best) *best = bv;
     if (ms_per_step)
-        for (int v = 0; v < nv; ++v) ms_per_step[v] = ms[(size_t(bc) * norders + bo) * nv + v];
+        for (int v = 0; v < nv; ++v) ms_per_step[v] = row[v];
     return LGF_OK;
 }
 
