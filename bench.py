@@ -436,14 +436,14 @@ def hold_flow(lgf, ctx, steps):
     f.close()
 
 
-def profile_run(which, steps):
+def profile_run(which, steps, order=404):
     """Fixed workload for rocprofv3 counter runs: `steps` RK3 steps of a
     seeded random 24^4 lattice in one precision (direct launches, tile order
-    404), two clover measurements, and one plain copy of a link array
-    (k_convert, a kernel of known traffic that calibrates the counters).  "fp32stream" runs the same
-    on a 32^4 lattice, which does not fit the last-level cache, with the
-    stage kernel and tile order the autotuner picks for it (the
-    shared-backward-link kernel with non-temporal stores, x0/x2 blocks of 4)."""
+    `order`, x0/x2 blocks of 4 by default), two clover measurements, and one
+    plain copy of a link array (k_convert, a kernel of known traffic that
+    calibrates the counters).  "fp32stream" runs the same on a 32^4 lattice,
+    which does not fit the last-level cache, with the shared-backward-link
+    stage kernel with non-temporal stores (the autotuner's pick there)."""
     from msc_project_amd import lgf
     ctx = lgf.Context(0)
     if which == "fp32hold":
@@ -451,7 +451,7 @@ def profile_run(which, steps):
         return
     prec = lgf.FP64 if which == "fp64" else lgf.FP32
     shape = STREAM_SHAPE if which == "fp32stream" else SHAPE
-    ctx.set_tile_order(404)  # the order the autotuner picks for these lattices (x0 and x2 blocks of 4)
+    ctx.set_tile_order(order)
     if which == "fp32pb":  # the shared-backward-link stage kernel
         ctx.set_stage_variant(lgf.FP32, lgf.FP32_COMPACT_VARIANTS)
     if which == "fp32stream":
@@ -468,7 +468,7 @@ def profile_run(which, steps):
     ctx.synchronize()
     stride = (np.prod(shape) + 63) // 64 * 64
     nbytes = 72 * stride * (8 if prec == lgf.FP64 else 4)
-    print(json.dumps({"profile": which, "lattice": list(shape), "steps": steps,
+    print(json.dumps({"profile": which, "lattice": list(shape), "steps": steps, "tile_order": order,
                       "copy_kernel_read_bytes": int(nbytes), "copy_kernel_write_bytes": int(nbytes),
                       "stage_algorithmic_bytes_per_step": BYTES_PER_LU[prec] * 4 * int(np.prod(shape))}))
 
@@ -518,9 +518,10 @@ def main():
                     help="requests of the comparison stream (every job steps); 0 disables it")
     ap.add_argument("--profile", choices=["fp32", "fp32pb", "fp32stream", "fp32hold", "fp64", "fp32native"], default=None,
                     help="run the fixed counter-profiling workload instead of the benchmark")
+    ap.add_argument("--tile-order", type=int, default=404, help="tile order of the --profile workload")
     args = ap.parse_args()
     if args.profile:
-        profile_run(args.profile, args.steps)
+        profile_run(args.profile, args.steps, args.tile_order)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the flow path runs one lattice (or one batch) per GPU "
