@@ -664,10 +664,11 @@ constexpr bool kCloverXLines = sizeof(S) == 8 ? LGF_CLOVER_XLINES : sizeof(C) ==
 // and 1 and their conjugated cross product.  Every line is then read as rows
 // 0 and 1 (kCloverRows, and k_lines stores only those), and where a product
 // needs more of it (a right factor, or columns 0, 1 of a left factor) its
-// third row is rebuilt: both sweeps over FP32 links are bound by the bytes
-// their loads return through the vector memory path (24^4, FP32 sweep:
-// texture data unit 88-91 % busy, VALU 54 %), and a rebuilt row costs 24
-// FMAs against 24 of 72 bytes (FP32) or 48 of 144 (FP64).  With FP64 links
+// third row is rebuilt: a rebuilt row costs 24 FMAs against 24 of 72 bytes
+// (FP32) or 48 of 144 (FP64) through the vector memory path.  With whole
+// lines the leaf-by-leaf FP32 sweep was bound by those bytes (24^4: texture
+// data unit 88-91 % busy, VALU 54 %); the factored sweep is near both limits
+// (VALU 96 %, texture data unit 89 % busy).  With FP64 links
 // (the parity path) the products are full, as the reference's: its links are
 // not projected, and rebuilding rows would move the result by the links'
 // drift from SU(3).
