@@ -1,5 +1,6 @@
-"""Build check of the compiled RK3 stage, line-product and clover kernels:
-every global memory access must take its address from a pointer.
+"""Build check of the compiled RK3 stage, line-product and clover kernels
+(k_clover, k_clover_pairs, k_clover_combine): every global memory access
+must take its address from a pointer.
 
 A shared-backward-link stage variant once faulted the GPU because the
 compiler merged the tails of the four direction cases of a `switch` into one
@@ -11,7 +12,9 @@ in _build/kernels.hip.o) and replays that path-by-path reading for every
 stage kernel and every direction a wave can take:
 
 - the direction (threadIdx.y or blockIdx.y: 0..3 for the stage and
-  line-product kernels, the plane 0..5 for the clover) is fixed per run, and
+  line-product kernels, the plane 0..5 for the clover, the plane of a pair
+  0..1 for k_clover_pairs, whose pair comes from blockIdx.x and is not
+  fixed) is fixed per run, and
   scalar constants, SCC, VCC and EXEC (zero / non-zero) are propagated, so
   that branches on the direction are decided and only the paths that a wave
   of that direction can take are followed (conditional constant
@@ -30,7 +33,9 @@ The stage kernels take (FieldDesc* fds, int2* tiles, int ping): kernel
 argument bytes 0..15 are the two pointers.  The line-product kernel takes
 (fds, tiles, int t0, int ping, C* lines, int64_t G) and the clover kernel
 (fds, tiles, t0, ping, lines, G, double* partials): pointers at bytes 0, 8,
-24 and 40.  FieldDesc's pointer fields (links[0], links[1], acc, qx) are its
+24 and 40; k_clover_pairs (fds, tiles, t0, ping, C* lines, G) at 0, 8 and 24;
+k_clover_combine (fds, tiles, t0, const C* lines, G, double* partials) at 0,
+8, 24 and 40 (t0 is followed by 4 bytes of padding).  FieldDesc's pointer fields (links[0], links[1], acc, qx) are its
 first 32 bytes.
 """
 import concurrent.futures
@@ -48,6 +53,8 @@ FAMILIES = (
     (re.compile(r"_ZN3lgf1[0-9]k_rk3_stage"), 4, (0, 8)),  # k_rk3_stage, k_rk3_stage_pl, k_rk3_stage_pb
     (re.compile(r"_ZN3lgf7k_lines"), 4, (0, 8, 24)),
     (re.compile(r"_ZN3lgf8k_clover"), 6, (0, 8, 24, 40)),
+    (re.compile(r"_ZN3lgf14k_clover_pairs"), 2, (0, 8, 24)),  # the plane also depends on blockIdx.x
+    (re.compile(r"_ZN3lgf16k_clover_combine"), 1, (0, 8, 24, 40)),
 )
 STAGE_FAMILY = FAMILIES[0][1:]
 # dword offsets of the pointer arguments of the kernel being checked (set by check_kernel)

@@ -600,6 +600,15 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 #ifndef LGF_CLOVER_ORDERED_INNER
 #define LGF_CLOVER_ORDERED_INNER 0
 #endif
+// LGF_CLOVER_PAIRS 1: the FP32 sweep runs k_clover_pairs (two planes per
+// workgroup) and k_clover_combine instead of k_clover;
+// LGF_CLOVER_WPE_PAIRS is the pair kernel's minimum waves per SIMD
+#ifndef LGF_CLOVER_PAIRS
+#define LGF_CLOVER_PAIRS 1
+#endif
+#ifndef LGF_CLOVER_WPE_PAIRS
+#define LGF_CLOVER_WPE_PAIRS 4
+#endif
 
 // Line products.  L^d_l(y) = U_d(y) U_d(y+d) ... U_d(y+(l-1)d), the product
 // of l links along a straight line.  k_lines stores L_2 and L_3 of every
@@ -621,7 +630,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 // LGF/Field/SU3_field.cpp:550-560).  The field strength depends on a clover
 // sum only through C - C^dag, to which W and -W^dag contribute alike.  Leaf
 // shapes come in both orientations (m x n and n x m) with equal weights.
-template <typename S, typename C>
+// XC: the lines that start at the workgroup's own sites are read from its
+// LDS cache (xl) rather than from the line buffer
+template <typename S, typename C, bool XC>
 struct Lines {
     const S* U;   // current links: lines of length 1
     int64_t st;   // link stride
@@ -685,8 +696,8 @@ constexpr int kXLen = kCloverRows<S, C> ? 12 : 18;
 
 // a line that starts at the workgroup's own site x, from the LDS cache:
 // whole, or (ROWS = 2) rows 0 and 1 where the cache holds only those
-template <int LEN, int ROWS, typename S, typename C>
-LGF_DEV M3<C> xline(const Lines<S, C>& L, int dir) {
+template <int LEN, int ROWS, typename S, typename C, bool XC>
+LGF_DEV M3<C> xline(const Lines<S, C, XC>& L, int dir) {
     constexpr int X = kXLen<S, C>;
     M3<C> m;
     const C* q = L.xl + (size_t)(4 * (LEN - 1) + dir) * X * kTileSites + L.lane;
@@ -699,8 +710,8 @@ LGF_DEV M3<C> xline(const Lines<S, C>& L, int dir) {
 // ROWS = 2: rows 0, 1; 3: whole; 4: columns 0, 1 (a left factor of
 // mul_dd_r01 or a right factor of mul_nn_c01), which for whole FP32 lines
 // are the four float4 planes.
-template <int LEN, int ROWS, typename S, typename C>
-LGF_DEV M3<C> line(const Lines<S, C>& L, int dir, int site) {
+template <int LEN, int ROWS, typename S, typename C, bool XC>
+LGF_DEV M3<C> line(const Lines<S, C, XC>& L, int dir, int site) {
     if constexpr (kCloverSu3<S> && (ROWS == 2 || kCloverRows<S, C>)) {
         M3<C> m = LEN == 1 ? load_link_rows01<S, C>(L.U, L.st, dir, site)
                            : load_link_rows01<C, C>(LEN == 2 ? L.L2 : L.L3, L.G, dir, L.lbase + site);
@@ -753,14 +764,14 @@ struct PlaneOff {
 // reads, 72 FMAs.
 
 // a whole line that starts at the workgroup's own site x
-template <int LEN, typename S, typename C>
-LGF_DEV M3<C> own_line(const Lines<S, C>& L, int dir, int s) {
-    if constexpr (kCloverXLines<S, C>) return xline<LEN, 3>(L, dir);
+template <int LEN, typename S, typename C, bool XC>
+LGF_DEV M3<C> own_line(const Lines<S, C, XC>& L, int dir, int s) {
+    if constexpr (XC) return xline<LEN, 3>(L, dir);
     else return line<LEN, 3>(L, dir, s);
 }
 
-template <int M, int N, typename S, typename C>
-LGF_DEV void leaves_a(M3<C>& Y, C w, bool first, C dep, const Lines<S, C>& L, int s, const PlaneOff& o, int mu,
+template <int M, int N, typename S, typename C, bool XC>
+LGF_DEV void leaves_a(M3<C>& Y, C w, bool first, C dep, const Lines<S, C, XC>& L, int s, const PlaneOff& o, int mu,
                       int nu) {
 #if LGF_CLOVER_ORDERED
     asm volatile("" : "+v"(s) : "v"(dep));
@@ -792,8 +803,8 @@ LGF_DEV void leaves_a(M3<C>& Y, C w, bool first, C dep, const Lines<S, C>& L, in
     axpy(Y, -w, p4);
 }
 
-template <int M, int N, typename S, typename C>
-LGF_DEV void leaves_b(M3<C>& Z, C w, bool first, C dep, const Lines<S, C>& L, int s, const PlaneOff& o, int mu,
+template <int M, int N, typename S, typename C, bool XC>
+LGF_DEV void leaves_b(M3<C>& Z, C w, bool first, C dep, const Lines<S, C, XC>& L, int s, const PlaneOff& o, int mu,
                       int nu) {
 #if LGF_CLOVER_ORDERED
     asm volatile("" : "+v"(s) : "v"(dep));
@@ -861,8 +872,8 @@ LGF_DEV M3<C> project_F(const Ta9<C>& t) {
 template <typename C, typename T>
 LGF_DEV void herm_pack(const M3<C>& F, T* h, int step);
 
-template <typename S, typename C, typename T>
-LGF_DEV C plane_F_factored(const Lines<S, C>& L, const PlaneOff& o, int s, int mu, int nu, M3<C>& Fimp, T* f11,
+template <typename S, typename C, typename T, bool XC>
+LGF_DEV C plane_F_factored(const Lines<S, C, XC>& L, const PlaneOff& o, int s, int mu, int nu, M3<C>& Fimp, T* f11,
                            int f11_step) {
     const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
     // per-leaf weights of the improved sum: k1/4 of the four square 1x1
@@ -908,8 +919,8 @@ LGF_DEV C plane_F_factored(const Lines<S, C>& L, const PlaneOff& o, int s, int m
     return e;
 }
 
-template <typename S, typename C, typename T>
-LGF_DEV C plane_F(const Lines<S, C>& L, const Geo& g, const double* rcp, int mu, int nu, M3<C>& Fimp, T* f11,
+template <typename S, typename C, typename T, bool XC>
+LGF_DEV C plane_F(const Lines<S, C, XC>& L, const Geo& g, const double* rcp, int mu, int nu, M3<C>& Fimp, T* f11,
                   int f11_step) {
     PlaneOff o;
 #pragma unroll
@@ -1017,7 +1028,7 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     const int s = tm.y * kTileSites + threadIdx.x;
     const int p = __builtin_amdgcn_readfirstlane(threadIdx.y);
     const bool live = s < d.vol;
-    Lines<S, C> L;
+    Lines<S, C, XL> L;
     L.U = (const S*)d.links[ping ^ d.flip];
     L.st = d.stride;
     L.L2 = lines;
@@ -1120,6 +1131,139 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
             o[3] = eis;
             o[4] = q11;
         }
+    }
+}
+
+// The FP32 sweep takes the planes in pairs: a workgroup of two waves
+// computes planes 2k and 2k+1 of one tile, the pairs whose products make
+// q(x), so that the field strengths of a pair still meet in LDS.  Its two
+// waves need no line cache and hold 11 KB of LDS, so a CU runs twenty of
+// these waves (90 VGPRs) instead of the eighteen of three six-wave
+// workgroups, and a barrier joins two waves instead of six: the 24^4 sweep
+// takes 0.453-0.464 ms instead of 0.479-0.485.  (The FP64-arithmetic sweep
+// over FP32 links gains nothing from the pairs, 0.92 ms either way, and
+// keeps k_clover.)  Each workgroup leaves its pair's q(x)
+// per site and its tile sums of e, ei and q11; k_clover_combine adds the
+// three pairs of every site in the order k_clover does and writes the same
+// tile partials (e, ei and q11 are summed per pair first, so their tile
+// sums can differ from k_clover's in the last bits).
+//
+// Both go in the part of the line buffer that lines stored as rows 0 and 1
+// leave free: elements [12 G, 18 G) of each direction block of 18 G (the
+// third-row planes), 24 bytes per site in FP32; the pair charges (3 G
+// doubles) take that of L_2 direction 0, the tile sums (9 doubles per tile)
+// that of L_2 direction 1.
+template <typename C>
+LGF_DEV double* pair_charges(C* lines, int64_t G) {
+    return (double*)(lines + 12 * G);
+}
+template <typename C>
+LGF_DEV double* pair_sums(C* lines, int64_t G) {
+    return (double*)(lines + 30 * G);
+}
+
+template <typename S, typename C, int WPE>
+__global__ __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(WPE, 8))) void k_clover_pairs(
+    const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles, int t0, int ping, C* __restrict__ lines,
+    int64_t G) {
+    static_assert(kCloverRows<S, C>, "the pair charges use the planes that rows-0-and-1 lines leave free");
+    using T = typename std::conditional<LGF_CLOVER_LDS_REAL != 0, C, double>::type;
+    __shared__ T sF[2][9][kTileSites];
+    __shared__ T sF11[2][9][kTileSites];
+    __shared__ double sE[2][kTileSites];
+    __shared__ double sEi[2][kTileSites];
+    const int item = xcd_remap(blockIdx.x, gridDim.x);
+    const int tix = item / 3, k = item - 3 * tix;
+    const int2 tm = tiles[tix];
+    const FieldDesc& d = fds[tm.x];
+    const int s = tm.y * kTileSites + threadIdx.x;
+    const int h = __builtin_amdgcn_readfirstlane(threadIdx.y);
+    const int p = 2 * k + h;
+    const bool live = s < d.vol;
+    Lines<S, C, false> L;
+    L.U = (const S*)d.links[ping ^ d.flip];
+    L.st = d.stride;
+    L.L2 = lines;
+    L.L3 = lines + 4 * 18 * G;
+    L.G = G;
+    L.lbase = (d.tile0 - t0) * kTileSites;
+    L.xl = nullptr;
+    L.lane = threadIdx.x;
+    double e = 0.0, ei = 0.0;
+    if (live) {
+        const Geo g = make_geo(d, s);
+        // plane p: (3,2),(0,1),(3,1),(2,0),(3,0),(1,2)
+        const int mu = (p & 1) ? (p == 1 ? 0 : (p == 3 ? 2 : 1)) : 3;
+        const int nu = (p & 1) ? (p == 1 ? 1 : (p == 3 ? 0 : 2)) : (p == 0 ? 2 : (p == 2 ? 1 : 0));
+        M3<C> Fi;
+        e = (double)plane_F<S, C, T>(L, g, d.rcp, mu, nu, Fi, &sF11[h][0][threadIdx.x], kTileSites);
+        herm_pack(Fi, &sF[h][0][threadIdx.x], kTileSites);
+        ei = (double)retr_mul(Fi, Fi);
+    } else {
+#pragma unroll
+        for (int i = 0; i < 9; ++i) sF[h][i][threadIdx.x] = sF11[h][i][threadIdx.x] = T(0);
+    }
+    sE[h][threadIdx.x] = e;
+    sEi[h][threadIdx.x] = ei;
+    __syncthreads();
+    if (h == 0) {
+        T a[9], b[9];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a[i] = sF[0][i][threadIdx.x], b[i] = sF[1][i][threadIdx.x];
+        const double q = herm_dot(a, b);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) a[i] = sF11[0][i][threadIdx.x], b[i] = sF11[1][i][threadIdx.x];
+        double q11 = herm_dot(a, b);
+        double es = sE[0][threadIdx.x] + sE[1][threadIdx.x];
+        double eis = sEi[0][threadIdx.x] + sEi[1][threadIdx.x];
+        pair_charges(lines, G)[k * G + L.lbase + s] = live ? q : 0.0;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+            es += __shfl_down(es, off, 64);
+            eis += __shfl_down(eis, off, 64);
+            q11 += __shfl_down(q11, off, 64);
+        }
+        if (threadIdx.x == 0) {
+            double* o = pair_sums(lines, G) + 9 * (d.tile0 - t0 + tm.y) + 3 * k;
+            o[0] = es;
+            o[1] = eis;
+            o[2] = q11;
+        }
+    }
+}
+
+// q(x) = the three pairs' charges of each site, |q|, and the tile partials
+// (e, q, |q|, ei, q11) from the pair sums
+template <typename C>
+__global__ __launch_bounds__(64) void k_clover_combine(const FieldDesc* __restrict__ fds, const int2* __restrict__ tiles,
+                                                       int t0, const C* __restrict__ lines, int64_t G,
+                                                       double* __restrict__ partials) {
+    const int tix = xcd_remap(blockIdx.x, gridDim.x);
+    const int2 tm = tiles[tix];
+    const FieldDesc& d = fds[tm.x];
+    const int s = tm.y * kTileSites + threadIdx.x;
+    const bool live = s < d.vol;
+    const double* qp = pair_charges(const_cast<C*>(lines), G) + (d.tile0 - t0) * kTileSites + s;
+    double qs = 0.0;
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) qs += qp[k * G];
+    }
+    if (live && d.qx) d.qx[s] = qs;
+    double qa = fabs(qs);
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        qs += __shfl_down(qs, off, 64);
+        qa += __shfl_down(qa, off, 64);
+    }
+    if (threadIdx.x == 0) {
+        const double* ps = pair_sums(const_cast<C*>(lines), G) + 9 * (d.tile0 - t0 + tm.y);
+        double* o = partials + kCloverParts * (d.tile0 + tm.y);
+        o[0] = ps[0] + ps[3] + ps[6];
+        o[1] = qs;
+        o[2] = qa;
+        o[3] = ps[1] + ps[4] + ps[7];
+        o[4] = ps[2] + ps[5] + ps[8];
     }
 }
 
@@ -1427,7 +1571,7 @@ hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2
                          void* lines, double* partials, hipStream_t stream) {
     const int2* ct = tiles + t0;
     const int64_t G = (int64_t)ntiles * kTileSites;
-    dim3 grid(ntiles), lblock(kTileSites, 4), block(kTileSites, 6);
+    dim3 grid(ntiles), lblock(kTileSites, 4), block(kTileSites, 6), pgrid(3 * ntiles), pblock(kTileSites, 2);
     if (prec == kPrecFP64) {
         hipLaunchKernelGGL((k_lines<double, double>), grid, lblock, 0, stream, fds, ct, t0, ping, (double*)lines, G);
         hipLaunchKernelGGL((k_clover<double, double, LGF_CLOVER_WPE64_PARITY>), grid, block, 0, stream, fds, ct, t0, ping,
@@ -1438,8 +1582,15 @@ hipError_t launch_clover(int prec, int compute, const FieldDesc* fds, const int2
                            (const double*)lines, G, partials);
     } else {
         hipLaunchKernelGGL((k_lines<float, float>), grid, lblock, 0, stream, fds, ct, t0, ping, (float*)lines, G);
-        hipLaunchKernelGGL((k_clover<float, float, LGF_CLOVER_WPE>), grid, block, 0, stream, fds, ct, t0, ping, (const float*)lines,
-                           G, partials);
+        if (LGF_CLOVER_PAIRS) {
+            hipLaunchKernelGGL((k_clover_pairs<float, float, LGF_CLOVER_WPE_PAIRS>), pgrid, pblock, 0, stream, fds, ct,
+                               t0, ping, (float*)lines, G);
+            hipLaunchKernelGGL((k_clover_combine<float>), grid, dim3(kTileSites), 0, stream, fds, ct, t0,
+                               (const float*)lines, G, partials);
+        } else {
+            hipLaunchKernelGGL((k_clover<float, float, LGF_CLOVER_WPE>), grid, block, 0, stream, fds, ct, t0, ping,
+                               (const float*)lines, G, partials);
+        }
     }
     return hipGetLastError();
 }
