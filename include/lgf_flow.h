@@ -134,7 +134,7 @@ int lgf_ctx_set_measure_scratch(lgf_ctx* ctx, size_t bytes);
 /* tile orders the autotuners time (lgf_ctx_set_tile_order codes) */
 #define LGF_AUTOTUNE_TILE_ORDERS {0, 2, 4, 8, 404, 408, 604, 1204, 6402, 6404}
 /* time every pair of tile order (LGF_AUTOTUNE_TILE_ORDERS) and stage variant for
- * nsteps RK3 steps on a seeded lattice of `shape` (each pair three times, its
+ * nsteps graph-replayed RK3 steps on a seeded lattice of `shape` (each pair three times, its
  * fastest run counting); time the four fastest pairs again with 3*nsteps
  * steps, then every variant at the order of the fastest of those, and make
  * that order and the fastest of those variants this context's tile order and

@@ -934,7 +934,7 @@ constexpr int kRefineSteps = 3;
 static int time_steps(lgf_ctx* ctx, lgf_batch* b, const double* eps, int nsteps, hipEvent_t e0, hipEvent_t e1,
                       float* ms) {
     if (hipEventRecord(e0, ctx->stream) != hipSuccess) return lgf_fail(ctx, LGF_E_HIP, "event record");
-    int rc = lgf_batch_flow_steps(b, eps, nsteps, 0);
+    int rc = lgf_batch_flow_steps(b, eps, nsteps, 1);
     if (rc) return rc;
     if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
         hipEventElapsedTime(ms, e0, e1) != hipSuccess)
@@ -942,13 +942,14 @@ static int time_steps(lgf_ctx* ctx, lgf_batch* b, const double* eps, int nsteps,
     return LGF_OK;
 }
 
-// fastest of kAutotuneReps timings of nsteps steps (after one untimed step)
-// of a batch of `fields` created with the context's current settings
+// fastest of kAutotuneReps timings of nsteps graph-replayed steps, as the
+// flow runs them (after two untimed steps, which capture the graphs), of a
+// batch of `fields` created with the context's current settings
 static int time_batch(lgf_ctx* ctx, const std::vector<lgf_field*>& fields, const std::vector<double>& eps,
                       int nsteps, hipEvent_t e0, hipEvent_t e1, double* ms_step) {
     lgf_batch* b = nullptr;
     int rc = lgf_batch_create(ctx, fields.data(), (int)fields.size(), &b);
-    if (!rc) rc = lgf_batch_flow_steps(b, eps.data(), 1, 0);
+    if (!rc) rc = lgf_batch_flow_steps(b, eps.data(), 2, 1);  // captures the batch's step graphs
     *ms_step = 1e300;
     for (int r = 0; r < kAutotuneReps && !rc; ++r) {
         float m = 0.f;
