@@ -1153,6 +1153,10 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 // third-row planes), 24 bytes per site in FP32; the pair charges (3 G
 // doubles) take that of L_2 direction 0, the tile sums (9 doubles per tile)
 // that of L_2 direction 1.
+// the free planes of a direction block hold 6 G reals of C: 3 G doubles of
+// pair charges, and 9 doubles per tile (9 G / 64) of tile sums
+static_assert(3 * sizeof(double) <= 6 * sizeof(float) && 9 * sizeof(double) <= 6 * sizeof(float) * kTileSites,
+              "pair charges and tile sums fit the free third-row planes");
 template <typename C>
 LGF_DEV double* pair_charges(C* lines, int64_t G) {
     return (double*)(lines + 12 * G);
