@@ -693,6 +693,19 @@ constexpr bool kCloverRows = LGF_CLOVER_ROWS2 && kCloverSu3<S>;
 // reals per line in the LDS line cache
 template <typename S, typename C>
 constexpr int kXLen = kCloverRows<S, C> ? 12 : 18;
+// With FP64 links the cache also holds five backward lines L^d_k(x - k d),
+// each staged by the lane of site x: (d, 1) for every d at slots 12 + d and
+// (3, 2) at slot 16 (157 KB in all).  A site's six planes read them 60 times
+// per site (R4 and the left factor of K2 along mu, g_n along nu): the most
+// used of the backward lines, direction 3 being the first direction of
+// three planes.
+#ifndef LGF_CLOVER_XBACK
+#define LGF_CLOVER_XBACK 1
+#endif
+template <typename S, typename C>
+constexpr bool kXBack = sizeof(S) == 8 && LGF_CLOVER_XLINES && LGF_CLOVER_XBACK;
+template <typename S, typename C>
+constexpr int kXSlots = kXBack<S, C> ? 17 : 12;
 
 // a line that starts at the workgroup's own site x, from the LDS cache:
 // whole, or (ROWS = 2) rows 0 and 1 where the cache holds only those
@@ -770,7 +783,39 @@ LGF_DEV M3<C> own_line(const Lines<S, C, XC>& L, int dir, int s) {
     else return line<LEN, 3>(L, dir, s);
 }
 
-template <int M, int N, typename S, typename C, bool XC>
+// the whole line L^dir_LEN(x - LEN dir), site = x - LEN dir: from the cache
+// where it holds that line (dir is wave-uniform; used for g_n along nu)
+template <int LEN, typename S, typename C, bool XC>
+LGF_DEV M3<C> back_line(const Lines<S, C, XC>& L, int dir, int site) {
+    if constexpr (XC && kXBack<S, C> && LEN <= 2) {
+        if (LEN == 1 || dir == 3) {
+            M3<C> m;
+            const C* q = L.xl + (size_t)(LEN == 1 ? 12 + dir : 16) * 18 * kTileSites + L.lane;
+#pragma unroll
+            for (int c = 0; c < 18; ++c) m.v[c] = q[c * kTileSites];
+            return m;
+        }
+    }
+    return line<LEN, 3>(L, dir, site);
+}
+
+// the same along mu in the planes whose mu is 3 (MU3), where the cache holds
+// L^3_1(x - e3) and L^3_2(x - 2 e3): a compile-time choice, since a branch
+// around the leaves' loads breaks their ordering and spills
+template <int LEN, bool MU3, typename S, typename C, bool XC>
+LGF_DEV M3<C> mu_back_line(const Lines<S, C, XC>& L, int mu, int site) {
+    if constexpr (MU3 && LEN <= 2) {
+        M3<C> m;
+        const C* q = L.xl + (size_t)(LEN == 1 ? 15 : 16) * 18 * kTileSites + L.lane;
+#pragma unroll
+        for (int c = 0; c < 18; ++c) m.v[c] = q[c * kTileSites];
+        return m;
+    } else {
+        return line<LEN, 3>(L, mu, site);
+    }
+}
+
+template <int M, int N, bool MU3, typename S, typename C, bool XC>
 LGF_DEV void leaves_a(M3<C>& Y, C w, bool first, C dep, const Lines<S, C, XC>& L, int s, const PlaneOff& o, int mu,
                       int nu) {
 #if LGF_CLOVER_ORDERED
@@ -798,12 +843,12 @@ LGF_DEV void leaves_a(M3<C>& Y, C w, bool first, C dep, const Lines<S, C, XC>& L
         p4 = mul_nn_su3(mid4, line<M, 3>(L, mu, s4));
     } else {
         const M3<C> h4 = mul_nn(line<N, 3>(L, nu, s4), line<M, 3>(L, mu, s4 + o.n[3 + N]));
-        p4 = mul_dn(h4, line<M, 3>(L, mu, s4));
+        p4 = mul_dn(h4, mu_back_line<M, MU3>(L, mu, s4));
     }
     axpy(Y, -w, p4);
 }
 
-template <int M, int N, typename S, typename C, bool XC>
+template <int M, int N, bool MU3, typename S, typename C, bool XC>
 LGF_DEV void leaves_b(M3<C>& Z, C w, bool first, C dep, const Lines<S, C, XC>& L, int s, const PlaneOff& o, int mu,
                       int nu) {
 #if LGF_CLOVER_ORDERED
@@ -833,7 +878,7 @@ LGF_DEV void leaves_b(M3<C>& Z, C w, bool first, C dep, const Lines<S, C, XC>& L
         const M3<C> k2 = mul_nn_c01(line<N, 3>(L, nu, sc), line<M, 4>(L, mu, smm));
         t2 = mul_dn_su3(k2, line<M, 3>(L, mu, sc));
     } else {
-        const M3<C> k2 = mul_nn(line<N, 3>(L, nu, sc), line<M, 3>(L, mu, smm));
+        const M3<C> k2 = mul_nn(line<N, 3>(L, nu, sc), mu_back_line<M, MU3>(L, mu, smm));
         t2 = mul_dn(k2, line<M, 3>(L, mu, sc));
     }
     axpy(Z, -w, t2);
@@ -872,7 +917,7 @@ LGF_DEV M3<C> project_F(const Ta9<C>& t) {
 template <typename C, typename T>
 LGF_DEV void herm_pack(const M3<C>& F, T* h, int step);
 
-template <typename S, typename C, typename T, bool XC>
+template <bool MU3, typename S, typename C, typename T, bool XC>
 LGF_DEV C plane_F_factored(const Lines<S, C, XC>& L, const PlaneOff& o, int s, int mu, int nu, M3<C>& Fimp, T* f11,
                            int f11_step) {
     const C k1 = C(19.0 / 9.0), k2 = C(1.0 / 36.0), k3 = C(-32.0 / 45.0), k4 = C(1.0 / 15.0);
@@ -884,26 +929,26 @@ LGF_DEV C plane_F_factored(const Lines<S, C, XC>& L, const PlaneOff& o, int s, i
     set_zero(acc);
     M3<C> Y;
     // quadrants (+nu,+mu), (-mu,+nu): TA(a_n Y_n)
-    leaves_a<1, 1>(Y, C(1), true, acc.v[8], L, s, o, mu, nu);
+    leaves_a<1, 1, MU3>(Y, C(1), true, acc.v[8], L, s, o, mu, nu);
     mul_nn_acc_ta(a11, own_line<1>(L, nu, s), Y);
-    leaves_a<2, 1>(Y, w12, true, a11.v[8], L, s, o, mu, nu);
-    leaves_a<3, 1>(Y, w13, false, Y.v[17], L, s, o, mu, nu);
+    leaves_a<2, 1, MU3>(Y, w12, true, a11.v[8], L, s, o, mu, nu);
+    leaves_a<3, 1, MU3>(Y, w13, false, Y.v[17], L, s, o, mu, nu);
     mul_nn_acc_ta(acc, own_line<1>(L, nu, s), Y);
-    leaves_a<1, 2>(Y, w12, true, acc.v[8], L, s, o, mu, nu);
-    leaves_a<2, 2>(Y, w22, false, Y.v[17], L, s, o, mu, nu);
+    leaves_a<1, 2, MU3>(Y, w12, true, acc.v[8], L, s, o, mu, nu);
+    leaves_a<2, 2, MU3>(Y, w22, false, Y.v[17], L, s, o, mu, nu);
     mul_nn_acc_ta(acc, own_line<2>(L, nu, s), Y);
-    leaves_a<1, 3>(Y, w13, true, acc.v[8], L, s, o, mu, nu);
+    leaves_a<1, 3, MU3>(Y, w13, true, acc.v[8], L, s, o, mu, nu);
     mul_nn_acc_ta(acc, own_line<3>(L, nu, s), Y);
     // quadrants (+mu,-nu), (-nu,-mu): TA(Z_n g_n)
-    leaves_b<1, 1>(Y, C(1), true, acc.v[8], L, s, o, mu, nu);
-    mul_nn_acc_ta(a11, Y, line<1, 3>(L, nu, s + o.n[2]));
-    leaves_b<2, 1>(Y, w12, true, a11.v[8], L, s, o, mu, nu);
-    leaves_b<3, 1>(Y, w13, false, Y.v[17], L, s, o, mu, nu);
-    mul_nn_acc_ta(acc, Y, line<1, 3>(L, nu, s + o.n[2]));
-    leaves_b<1, 2>(Y, w12, true, acc.v[8], L, s, o, mu, nu);
-    leaves_b<2, 2>(Y, w22, false, Y.v[17], L, s, o, mu, nu);
-    mul_nn_acc_ta(acc, Y, line<2, 3>(L, nu, s + o.n[1]));
-    leaves_b<1, 3>(Y, w13, true, acc.v[8], L, s, o, mu, nu);
+    leaves_b<1, 1, MU3>(Y, C(1), true, acc.v[8], L, s, o, mu, nu);
+    mul_nn_acc_ta(a11, Y, back_line<1>(L, nu, s + o.n[2]));
+    leaves_b<2, 1, MU3>(Y, w12, true, a11.v[8], L, s, o, mu, nu);
+    leaves_b<3, 1, MU3>(Y, w13, false, Y.v[17], L, s, o, mu, nu);
+    mul_nn_acc_ta(acc, Y, back_line<1>(L, nu, s + o.n[2]));
+    leaves_b<1, 2, MU3>(Y, w12, true, acc.v[8], L, s, o, mu, nu);
+    leaves_b<2, 2, MU3>(Y, w22, false, Y.v[17], L, s, o, mu, nu);
+    mul_nn_acc_ta(acc, Y, back_line<2>(L, nu, s + o.n[1]));
+    leaves_b<1, 3, MU3>(Y, w13, true, acc.v[8], L, s, o, mu, nu);
     mul_nn_acc_ta(acc, Y, line<3, 3>(L, nu, s + o.n[0]));
     C e;
     {
@@ -928,7 +973,10 @@ LGF_DEV C plane_F(const Lines<S, C, XC>& L, const Geo& g, const double* rcp, int
         o.m[3 + k] = k == 0 ? 0 : wrap_offset(g, mu, k, rcp);
         o.n[3 + k] = k == 0 ? 0 : wrap_offset(g, nu, k, rcp);
     }
-    return plane_F_factored<S, C, T>(L, o, g.s, mu, nu, Fimp, f11, f11_step);
+    if constexpr (kXBack<S, C> && XC) {
+        if (mu == 3) return plane_F_factored<true, S, C, T>(L, o, g.s, mu, nu, Fimp, f11, f11_step);
+    }
+    return plane_F_factored<false, S, C, T>(L, o, g.s, mu, nu, Fimp, f11, f11_step);
 }
 
 // Re Tr(A B) for Hermitian A, B given as 9 reals each:
@@ -1015,7 +1063,7 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
     };
     union Shared {
         Out o;
-        C xl[XL ? 12 : 1][kXLen<S, C>][kTileSites];
+        C xl[XL ? kXSlots<S, C> : 1][kXLen<S, C>][kTileSites];
     };
     __shared__ Shared sh;
     T (*sF)[9][kTileSites] = sh.o.F;
@@ -1052,6 +1100,18 @@ __global__ __launch_bounds__(384) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
                               : load_link<C, C>(k < 8 ? L.L2 : L.L3, G, dir, L.lbase + s);
 #pragma unroll
                 for (int c = 0; c < kXLen<S, C>; ++c) sh.xl[k][c][threadIdx.x] = m.v[c];
+            }
+            if constexpr (kXBack<S, C>) {
+                // wave p < 5 stages backward line 12 + p
+                if (p < 5) {
+                    const Geo g = make_geo(d, s);
+                    const int dir = p < 4 ? p : 3, len = p < 4 ? 1 : 2;
+                    const int sb = s + wrap_offset(g, dir, -len, d.rcp);
+                    const M3<C> m = len == 1 ? load_link<S, C>(L.U, L.st, dir, sb)
+                                             : load_link<C, C>(L.L2, G, dir, L.lbase + sb);
+#pragma unroll
+                    for (int c = 0; c < 18; ++c) sh.xl[12 + p][c][threadIdx.x] = m.v[c];
+                }
             }
         }
         __syncthreads();
