@@ -649,15 +649,16 @@ struct Lines {
 // where the six plane waves of the site read them: a fifth of the sweep's
 // line loads (a_n, R1 and the first factor of T3 in plane_F_factored).  That
 // kernel is bound by its line loads (VALU about a third busy) and holds one
-// workgroup per CU at its register budget, so the 110 KB of LDS cost no
-// occupancy; the cache shares its LDS with the per-plane field strengths,
-// which are written after every wave is done with it.  Without it the
-// sweep is 9 % slower (24^4).
-//   LGF_CLOVER_XLINES_ROWS: the same cache for the FP32 sweep, holding rows
-//   0 and 1 of each line (37 KB, within the LDS that sweep already uses for
-//   its field strengths, so it costs no residency): that sweep is bound by
-//   the bytes its loads return through the vector memory path, and the
-//   cache makes it 2-3 % faster (24^4).  The FP64-arithmetic sweep over FP32
+// workgroup per CU at its register budget, so the LDS (110 KB, 157 KB with
+// the backward lines below) costs no occupancy; the cache shares its LDS
+// with the per-plane field strengths, which are written after every wave is
+// done with it.  Without it the sweep is 9 % slower (24^4).
+//   LGF_CLOVER_XLINES_ROWS: the same cache for the six-plane FP32 kernel,
+//   holding rows 0 and 1 of each line (37 KB, within the LDS that kernel
+//   already uses for its field strengths, so it costs no residency): that
+//   kernel was bound by the bytes its loads return through the vector
+//   memory path, and the cache made it 2-3 % faster (24^4); the FP32 sweep
+//   now runs k_clover_pairs, which has no cache.  The FP64-arithmetic sweep over FP32
 //   links does not use it: with it that kernel needs fewer registers and two
 //   workgroups share a CU, which thrashes its caches (33 % slower), and at
 //   its old residency (LDS padded) it is no faster.
