@@ -1223,8 +1223,16 @@ LGF_DEV double* pair_charges(C* lines, int64_t G) {
     return (double*)(lines + 12 * G);
 }
 template <typename C>
+LGF_DEV const double* pair_charges(const C* lines, int64_t G) {
+    return (const double*)(lines + 12 * G);
+}
+template <typename C>
 LGF_DEV double* pair_sums(C* lines, int64_t G) {
     return (double*)(lines + 30 * G);
+}
+template <typename C>
+LGF_DEV const double* pair_sums(const C* lines, int64_t G) {
+    return (const double*)(lines + 30 * G);
 }
 
 template <typename S, typename C, int WPE>
@@ -1308,7 +1316,7 @@ __global__ __launch_bounds__(64) void k_clover_combine(const FieldDesc* __restri
     const FieldDesc& d = fds[tm.x];
     const int s = tm.y * kTileSites + threadIdx.x;
     const bool live = s < d.vol;
-    const double* qp = pair_charges(const_cast<C*>(lines), G) + (d.tile0 - t0) * kTileSites + s;
+    const double* qp = pair_charges(lines, G) + (d.tile0 - t0) * kTileSites + s;
     double qs = 0.0;
     if (live) {
 #pragma unroll
@@ -1322,7 +1330,7 @@ __global__ __launch_bounds__(64) void k_clover_combine(const FieldDesc* __restri
         qa += __shfl_down(qa, off, 64);
     }
     if (threadIdx.x == 0) {
-        const double* ps = pair_sums(const_cast<C*>(lines), G) + 9 * (d.tile0 - t0 + tm.y);
+        const double* ps = pair_sums(lines, G) + 9 * (d.tile0 - t0 + tm.y);
         double* o = partials + kCloverParts * (d.tile0 + tm.y);
         o[0] = ps[0] + ps[3] + ps[6];
         o[1] = qs;
