@@ -154,7 +154,14 @@ struct PlaqLds {
 // of 16, 17; FP64 nine double2 planes.  With LOC each direction wave loads
 // its own link once and leaves it here; the plaquettes read the rows they
 // need (0 and 1 of the left factor of each half-plaquette in FP32, all of it
-// in FP64) and the wave rereads its whole link for the update.
+// in FP64) and the wave rereads its whole link for the update.  In FP64
+// (LGF_FP64_BACK3) a fifth slot holds U_3(x - e3), which the lower terms of
+// the links of directions 0, 1 and 2 in their (mu,3) planes all read: the
+// direction-3 wave loads it once instead of three waves each loading it
+// (the workgroup's LDS grows from 64 to 74 KB, two workgroups per CU still).
+#ifndef LGF_FP64_BACK3
+#define LGF_FP64_BACK3 1
+#endif
 template <typename C>
 struct LocLds;
 template <>
@@ -164,7 +171,7 @@ struct LocLds<float> {
 };
 template <>
 struct LocLds<double> {
-    vd2 q[4][9][kTileSites];
+    vd2 q[4 + LGF_FP64_BACK3][9][kTileSites];
 };
 
 // Everything of link (x,mu) after its flow gradient is known (nine-real
@@ -337,7 +344,15 @@ LGF_DEV void stage_pre_pl(PlaqLds<C>& sp, const S* __restrict__ U, int64_t st, G
     after<ORDERED>(g, A.v[17]);
     lower_term<S, C, MU, N1>(A, U, st, g, fmu);
     after<ORDERED>(g, A.v[17]);
-    lower_term<S, C, MU, N2>(A, U, st, g, fmu);
+    if constexpr (N2 == 3 && LOC && sizeof(S) == 8 && LGF_FP64_BACK3) {
+        // lower_term<S, C, MU, 3> with U_3(x - e3) from the LDS slot
+        const int bnu = nb_off(g, 3, -1);
+        const M3<C> tt =
+            mul_halfstaple(load_link<S, C>(U, st, MU, g.s + bnu), load_link<S, C>(U, st, 3, g.s + bnu + fmu));
+        mul_dn_acc(A, tt, loc_load(*loc, 4, lane));
+    } else {
+        lower_term<S, C, MU, N2>(A, U, st, g, fmu);
+    }
     Umu = local_link<LOC, S, C>(U, st, MU, g.s, loc, lane);
     Ta9<C> w;
     set_zero(w);
@@ -385,6 +400,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE, 8))) v
 
     if constexpr (LOC) {
         loc_store(*loc, mu, lane, load_link<S, C>(U, st, mu, s));
+        if constexpr (sizeof(S) == 8 && LGF_FP64_BACK3) {
+            if (mu == 3) loc_store(*loc, 4, lane, load_link<S, C>(U, st, 3, s + nb_off(g, 3, -1)));
+        }
         __syncthreads();
     }
     C z[9];
