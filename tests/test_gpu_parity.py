@@ -405,7 +405,7 @@ def test_stage_variants_agree_and_autotune(ctx, shape):
         ctx.set_stage_variant(prec, 0)
     # the traversal order changes neither links nor observables
     res = []
-    for order in (0, 2, 4, 404, 203):
+    for order in (0, 203) + tuple(lgf.AUTOTUNE_TILE_ORDERS):
         ctx.set_tile_order(order)
         f = lgf.Field(ctx, shape, lgf.FP64)
         f.upload(U)
