@@ -118,6 +118,28 @@ def test_config1_golden_fp64(ctx, config1):
         assert q_err(o["Q_unimproved"], c["measured_Q_unimproved"], o["Qabs"]) < TOL_FP64
 
 
+@pytest.mark.parametrize("shape", [(6, 6, 6, 6), (4, 8, 2, 6)])
+def test_fp32_sweep_matches_fp64_arithmetic_sweep(ctx, shape):
+    """the FP32 sweep (clover planes in pairs, k_clover_pairs + k_clover_combine)
+    against the FP64-arithmetic sweep (six planes per workgroup, k_clover) on
+    the same FP32 links of a seeded random field flowed five steps: q(x) at
+    every site and every observable agree to FP32 rounding (6^4 has a ragged
+    last tile)"""
+    f = lgf.Field(ctx, shape, lgf.FP32)
+    f.set_random(77)
+    f.flow(0.02, 5)
+    o32 = f.observables(lgf.MEASURE_DEFAULT, want_q=True)
+    o64 = f.observables(lgf.MEASURE_FP64, want_q=True)
+    f.close()
+    qscale = np.max(np.abs(o64["qraw"]))
+    assert qscale > 0
+    assert np.max(np.abs(o32["qraw"] - o64["qraw"])) < 2e-5 * qscale
+    for k in ("E", "E_improved", "Qabs"):
+        assert rel_err(o32[k], o64[k]) < 2e-5, k
+    for k in ("Q", "Q_unimproved"):
+        assert abs(o32[k] - o64[k]) < 2e-5 * o64["Qabs"], k
+
+
 @pytest.mark.parametrize("shape,n01,n23", [((8, 8, 6, 6), 1, 1), ((6, 4, 4, 6), 1, -2), ((12, 12, 8, 8), 2, 1)])
 def test_twisted_flux_closed_form(ctx, shape, n01, n23):
     """the clover kernels against what the mathematics fixes: integer flux
